@@ -1,0 +1,556 @@
+// C ABI of the plbpd HIP library: state lifecycle, one entry point per
+// reference operator call, fused stepping, HIP-graph replay, timers.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "state.hpp"
+
+namespace plb {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+std::vector<double> tg_tables(int X, int Y, int Z) {
+    std::vector<double> t(2 * (size_t)(X + Y + Z));
+    double* p = t.data();
+    const int L[3] = {X, Y, Z};
+    for (int a = 0; a < 3; ++a) {
+        const double k = 2.0 * M_PI / (double)L[a];
+        for (int i = 0; i < L[a]; ++i) p[i] = std::sin(k * ((double)i + 0.5));
+        for (int i = 0; i < L[a]; ++i) p[L[a] + i] = std::cos(k * ((double)i + 0.5));
+        p += 2 * L[a];
+    }
+    return t;
+}
+
+int timer_begin(plb_state* s, int stage, TimerPair* tp) {
+    tp->stage = -1;
+    if (!s->timers_on) return PLB_OK;
+    auto take = [&](hipEvent_t* e) -> int {
+        if (!s->free_events.empty()) {
+            *e = s->free_events.back();
+            s->free_events.pop_back();
+            return PLB_OK;
+        }
+        PLB_HIP_TRY(hipEventCreate(e));
+        return PLB_OK;
+    };
+    int rc = take(&tp->start);
+    if (rc) return rc;
+    rc = take(&tp->stop);
+    if (rc) return rc;
+    tp->stage = stage;
+    PLB_HIP_TRY(hipEventRecord(tp->start, s->stream));
+    return PLB_OK;
+}
+
+namespace {
+int resolve_timers(plb_state* s, size_t keep) {
+    size_t n = s->pending.size() > keep ? s->pending.size() - keep : 0;
+    for (size_t i = 0; i < n; ++i) {
+        TimerPair& tp = s->pending[i];
+        PLB_HIP_TRY(hipEventSynchronize(tp.stop));
+        float ms = 0.f;
+        PLB_HIP_TRY(hipEventElapsedTime(&ms, tp.start, tp.stop));
+        s->stage_ms[tp.stage] += ms;
+        s->free_events.push_back(tp.start);
+        s->free_events.push_back(tp.stop);
+    }
+    s->pending.erase(s->pending.begin(), s->pending.begin() + n);
+    return PLB_OK;
+}
+}  // namespace
+
+int timer_end(plb_state* s, TimerPair* tp) {
+    if (tp->stage < 0) return PLB_OK;
+    PLB_HIP_TRY(hipEventRecord(tp->stop, s->stream));
+    s->pending.push_back(*tp);
+    if (s->pending.size() > 1024) return resolve_timers(s, 256);
+    return PLB_OK;
+}
+
+namespace {
+
+int enter(plb_state* s) {
+    if (!s) return fail(PLB_ERR_INVALID_ARGUMENT, "null state");
+    PLB_HIP_TRY(hipSetDevice(s->device));
+    return PLB_OK;
+}
+
+void drop_graph(plb_state* s) {
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_steps = 0;
+}
+
+template <class Fn>
+int timed(plb_state* s, int stage, Fn&& fn) {
+    TimerPair tp;
+    int rc = timer_begin(s, stage, &tp);
+    if (rc) return rc;
+    rc = fn();
+    const int rc2 = timer_end(s, &tp);
+    return rc ? rc : rc2;
+}
+
+bool same_bc(const plb_boundary& a, const plb_boundary& b) {
+    return std::memcmp(&a, &b, sizeof(plb_boundary)) == 0;
+}
+
+int ensure_fused_bc(plb_state* s, const plb_boundary* bc, std::vector<uint8_t>* mask) {
+    if (s->bc_set && same_bc(s->bc, *bc)) return PLB_OK;
+    drop_graph(s);
+    return fused_build_flags(s, bc, mask && !mask->empty() ? mask->data() : nullptr);
+}
+
+}  // namespace
+}  // namespace plb
+
+using namespace plb;
+
+extern "C" {
+
+const char* plb_last_error(void) { return g_error.c_str(); }
+
+int plb_version(void) { return 100; }
+
+int plb_device_info(int device, char* name, size_t name_len, int* num_cu, size_t* hbm_bytes) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(PLB_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= n) return fail(PLB_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    hipDeviceProp_t p;
+    PLB_HIP_TRY(hipGetDeviceProperties(&p, device));
+    if (name && name_len) {
+        std::strncpy(name, p.name, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (num_cu) *num_cu = p.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+    return PLB_OK;
+}
+
+int plb_lattice_tables(int q, int32_t* c, double* w, uint32_t* rev) {
+    auto fill = [&](auto lat) {
+        using L = decltype(lat);
+        for (int k = 0; k < L::q; ++k) {
+            if (c) {
+                c[3 * k] = L::cx(k);
+                c[3 * k + 1] = L::cy(k);
+                c[3 * k + 2] = L::cz(k);
+            }
+            if (w) w[k] = L::w(k);
+            if (rev) rev[k] = (uint32_t)plb::rev<L::q>(k);
+        }
+        return PLB_OK;
+    };
+    switch (q) {
+        case 9: return fill(Lat<9>());
+        case 19: return fill(Lat<19>());
+        case 27: return fill(Lat<27>());
+        default: return fail(PLB_ERR_LOGIC, "Model with q=" + std::to_string(q) + " not yet implemented");
+    }
+}
+
+int plb_state_create(const plb_state_desc* d, plb_state** out) {
+    if (!d || !out) return fail(PLB_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (d->q != 9 && d->q != 19 && d->q != 27)
+        return fail(PLB_ERR_LOGIC, "Model with q=" + std::to_string(d->q) + " not yet implemented");
+    if (d->x <= 0 || d->y <= 0 || d->z <= 0) return fail(PLB_ERR_INVALID_ARGUMENT, "argument is zero");
+    if (d->mode != PLB_MODE_LITERAL && d->mode != PLB_MODE_FUSED)
+        return fail(PLB_ERR_INVALID_ARGUMENT, "unknown mode");
+    if (storage_bytes(d->variant) == 0) return fail(PLB_ERR_INVALID_ARGUMENT, "unknown storage variant");
+    if (d->mode == PLB_MODE_LITERAL && d->variant != kF64 && d->variant != kF32)
+        return fail(PLB_ERR_INVALID_ARGUMENT, "literal mode supports PLB_F64 and PLB_F32 only");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PLB_ERR_NO_DEVICE, "no HIP device");
+    if (d->device < 0 || d->device >= ndev) return fail(PLB_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    PLB_HIP_TRY(hipSetDevice(d->device));
+
+    plb_state* s = new plb_state();
+    s->q = d->q;
+    s->X = d->x;
+    s->Y = d->y;
+    s->Z = d->z;
+    s->variant = d->variant;
+    s->mode = d->mode;
+    s->device = d->device;
+    s->sbytes = storage_bytes(d->variant);
+    s->cur = 0;
+    s->buf[0] = s->buf[1] = nullptr;
+    s->rho = s->u = nullptr;
+    s->flags = nullptr;
+    s->nbmask = nullptr;
+    s->has_solids = false;
+    s->bc_set = false;
+    std::memset(&s->bc, 0, sizeof(s->bc));
+    s->tau = 1.0;
+    s->force[0] = s->force[1] = s->force[2] = 0.0;
+    s->configured = false;
+    s->collide_pending = false;
+    s->moments_valid = false;
+    s->scratch = nullptr;
+    s->scratch_bytes = 0;
+    s->graph = nullptr;
+    s->graph_steps = 0;
+    s->graph_parity = 0;
+    s->timers_on = false;
+    for (double& v : s->stage_ms) v = 0.0;
+
+    const uint64_t X = d->x, Y = d->y, Z = d->z;
+    size_t mom_bytes;
+    if (d->mode == PLB_MODE_LITERAL) {
+        s->ncell = (X + 2) * (Y + 2) * (Z + 2);
+        s->pstride = s->ncell;
+        mom_bytes = s->ncell * s->sbytes;
+    } else {
+        s->ncell = X * Y * Z;
+        const size_t pad = d->pop_pad >= 0 ? (size_t)d->pop_pad : (size_t)(2048 + 64);
+        s->pstride = ((s->ncell + 63) / 64) * 64 + pad;
+        mom_bytes = s->ncell * sizeof(double);
+    }
+    auto cleanup = [&](int code, const std::string& msg) {
+        plb_state_destroy(s);
+        return fail(code, msg);
+    };
+    hipError_t e = hipStreamCreate(&s->stream);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(PLB_ERR_HIP, hipGetErrorString(e));
+    }
+    const size_t pop_bytes = s->pstride * s->q * s->sbytes;
+    for (int b = 0; b < 2; ++b) {
+        e = hipMalloc(&s->buf[b], pop_bytes);
+        if (e != hipSuccess) return cleanup(PLB_ERR_OOM, "population buffer: " + std::string(hipGetErrorString(e)));
+        e = hipMemset(s->buf[b], 0, pop_bytes);
+        if (e != hipSuccess) return cleanup(PLB_ERR_HIP, hipGetErrorString(e));
+    }
+    e = hipMalloc(&s->rho, mom_bytes);
+    if (e == hipSuccess) e = hipMalloc(&s->u, 3 * mom_bytes);
+    if (e == hipSuccess) e = hipMemset(s->rho, 0, mom_bytes);
+    if (e == hipSuccess) e = hipMemset(s->u, 0, 3 * mom_bytes);
+    if (e == hipSuccess && d->mode == PLB_MODE_FUSED) {
+        e = hipMalloc(&s->flags, s->ncell);
+        if (e == hipSuccess) e = hipMemset(s->flags, 0, s->ncell);
+    }
+    if (e != hipSuccess) return cleanup(PLB_ERR_OOM, hipGetErrorString(e));
+    *out = s;
+    return PLB_OK;
+}
+
+int plb_state_destroy(plb_state* s) {
+    if (!s) return PLB_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    for (auto& tp : s->pending) {
+        (void)hipEventDestroy(tp.start);
+        (void)hipEventDestroy(tp.stop);
+    }
+    for (auto e : s->free_events) (void)hipEventDestroy(e);
+    void* ptrs[] = {s->buf[0], s->buf[1], s->rho, s->u, s->flags, s->nbmask, s->scratch};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return PLB_OK;
+}
+
+int plb_state_bytes(const plb_state* s, size_t* bytes) {
+    if (!s || !bytes) return fail(PLB_ERR_INVALID_ARGUMENT, "null argument");
+    size_t b = 2 * s->pstride * s->q * s->sbytes;
+    if (s->mode == PLB_MODE_LITERAL)
+        b += 4 * s->ncell * s->sbytes;
+    else
+        b += 4 * s->ncell * sizeof(double) + s->ncell + (s->nbmask ? 4 * s->ncell : 0);
+    *bytes = b + s->scratch_bytes;
+    return PLB_OK;
+}
+
+int plb_upload(plb_state* s, int field, int layout, int dtype, const void* host) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!host) return fail(PLB_ERR_INVALID_ARGUMENT, "null host pointer");
+    if (dtype != PLB_DTYPE_F64 && dtype != PLB_DTYPE_F32) return fail(PLB_ERR_INVALID_ARGUMENT, "unknown dtype");
+    if (layout != PLB_LAYOUT_REFERENCE && layout != PLB_LAYOUT_COMPACT)
+        return fail(PLB_ERR_INVALID_ARGUMENT, "unknown layout");
+    drop_graph(s);
+    if (s->mode == PLB_MODE_LITERAL) return lit_transfer(s, field, layout, dtype, const_cast<void*>(host), true);
+    return fused_transfer(s, field, layout, dtype, const_cast<void*>(host), true);
+}
+
+int plb_download(plb_state* s, int field, int layout, int dtype, void* host) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!host) return fail(PLB_ERR_INVALID_ARGUMENT, "null host pointer");
+    if (dtype != PLB_DTYPE_F64 && dtype != PLB_DTYPE_F32) return fail(PLB_ERR_INVALID_ARGUMENT, "unknown dtype");
+    if (layout != PLB_LAYOUT_REFERENCE && layout != PLB_LAYOUT_COMPACT)
+        return fail(PLB_ERR_INVALID_ARGUMENT, "unknown layout");
+    if (s->mode == PLB_MODE_LITERAL) return lit_transfer(s, field, layout, dtype, host, false);
+    return fused_transfer(s, field, layout, dtype, host, false);
+}
+
+int plb_download_raw(plb_state* s, void* host, size_t bytes) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (s->mode != PLB_MODE_FUSED) return fail(PLB_ERR_LOGIC, "raw download is for the fused layout");
+    return fused_download_raw(s, host, bytes);
+}
+
+int plb_domain_initialize(plb_state* s, double rho0, const double u0[3], const plb_init_spec* init,
+                          const plb_boundary* bc) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!u0) return fail(PLB_ERR_INVALID_ARGUMENT, "null u0");
+    drop_graph(s);
+    double* tg = nullptr;
+    if (init && init->kind == PLB_INIT_TAYLOR_GREEN) {
+        const std::vector<double> t = tg_tables(s->X, s->Y, s->Z);
+        PLB_HIP_TRY(hipMalloc(&tg, t.size() * sizeof(double)));
+        PLB_HIP_TRY(hipMemcpy(tg, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else if (init && init->kind != PLB_INIT_UNIFORM && init->kind != PLB_INIT_PERTURBED) {
+        return fail(PLB_ERR_INVALID_ARGUMENT, "unknown initial-state kind");
+    }
+    rc = s->mode == PLB_MODE_LITERAL ? lit_initialize(s, rho0, u0, init, tg) : fused_initialize(s, rho0, u0, init, tg);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (tg) (void)hipFree(tg);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(PLB_ERR_HIP, hipGetErrorString(e));
+    s->moments_valid = false;
+    s->collide_pending = false;
+    if (bc) return plb_reset(s, bc);
+    return PLB_OK;
+}
+
+int plb_set_boundary(plb_state* s, const plb_boundary* bc, const uint8_t* mask) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
+    if (s->mode != PLB_MODE_FUSED) {
+        if (mask) return fail(PLB_ERR_LOGIC, "interior solids need fused mode");
+        s->bc = *bc;
+        s->bc_set = true;
+        return PLB_OK;
+    }
+    std::vector<uint8_t>& m = s->mask_host;
+    if (mask)
+        m.assign(mask, mask + s->ncell);
+    else
+        m.clear();
+    drop_graph(s);
+    s->bc_set = false;
+    return fused_build_flags(s, bc, mask);
+}
+
+int plb_collide_srt(plb_state* s, double tau, const double force[3]) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
+    if (s->mode == PLB_MODE_LITERAL)
+        return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, tau, force); });
+    if (s->tau != tau || std::memcmp(s->force, force, sizeof(s->force)) != 0) drop_graph(s);
+    s->tau = tau;
+    for (int i = 0; i < 3; ++i) s->force[i] = force[i];
+    s->collide_pending = true;
+    return PLB_OK;
+}
+
+int plb_stream(plb_state* s, const char* stream_type) {
+    int rc = enter(s);
+    if (rc) return rc;
+    const std::string t = stream_type ? stream_type : "";
+    if (t == "pull") return fail(PLB_ERR_LOGIC, "Streaming::_pull_tbb has not yet been implemented");
+    if (t != "push") return fail(PLB_ERR_LOGIC, "Invalid stream type: " + t);
+    if (s->mode == PLB_MODE_LITERAL) return timed(s, PLB_STAGE_STREAM, [&] { return lit_push(s); });
+    const bool collide = s->collide_pending;
+    s->collide_pending = false;
+    return timed(s, PLB_STAGE_FUSED, [&] { return fused_launch(s, collide); });
+}
+
+int plb_apply_noslip(plb_state* s, const plb_boundary* bc) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
+    if (s->mode == PLB_MODE_LITERAL) return timed(s, PLB_STAGE_NOSLIP, [&] { return lit_noslip(s, bc); });
+    // folded into the fused kernel; the boundary must be the one it uses
+    for (int f = 0; f < 6; ++f)
+        if (bc->type[f] == kNoslip && !bc->has_vel[f])
+            return fail(PLB_ERR_OUT_OF_RANGE, "no-slip face without a prescribed velocity");
+    return ensure_fused_bc(s, bc, &s->mask_host);
+}
+
+int plb_apply_periodicity(plb_state* s, const plb_boundary* bc) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
+    if (s->mode == PLB_MODE_LITERAL) return timed(s, PLB_STAGE_PERIODIC, [&] { return lit_periodic(s, bc); });
+    return ensure_fused_bc(s, bc, &s->mask_host);
+}
+
+int plb_calc_moments(plb_state* s) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (s->mode == PLB_MODE_LITERAL) return timed(s, PLB_STAGE_MOMENTS, [&] { return lit_moments(s); });
+    s->moments_valid = false;  // computed on demand from the populations
+    return PLB_OK;
+}
+
+int plb_reset_velocity(plb_state* s, const plb_boundary* bc) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
+    if (s->mode == PLB_MODE_LITERAL)
+        return timed(s, PLB_STAGE_RESET_VELOCITY, [&] { return lit_reset_velocity(s, bc); });
+    return ensure_fused_bc(s, bc, &s->mask_host);
+}
+
+int plb_reset_density(plb_state* s, const plb_boundary* bc) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
+    if (s->mode == PLB_MODE_LITERAL)
+        return timed(s, PLB_STAGE_RESET_DENSITY, [&] { return lit_reset_density(s, bc); });
+    return ensure_fused_bc(s, bc, &s->mask_host);
+}
+
+int plb_reset(plb_state* s, const plb_boundary* bc) {
+    int rc = plb_reset_velocity(s, bc);
+    if (rc) return rc;
+    return plb_reset_density(s, bc);
+}
+
+int plb_configure(plb_state* s, double tau, const double force[3], const plb_boundary* bc) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!force || !bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(tau > 0.5)) return fail(PLB_ERR_INVALID_ARGUMENT, "tau must exceed 1/2");
+    if (s->tau != tau || std::memcmp(s->force, force, sizeof(s->force)) != 0) drop_graph(s);
+    s->tau = tau;
+    for (int i = 0; i < 3; ++i) s->force[i] = force[i];
+    if (s->mode == PLB_MODE_FUSED) {
+        rc = ensure_fused_bc(s, bc, &s->mask_host);
+        if (rc) return rc;
+    } else {
+        for (int f = 0; f < 6; ++f)
+            if (bc->type[f] == kNoslip && !bc->has_vel[f])
+                return fail(PLB_ERR_OUT_OF_RANGE, "no-slip face without a prescribed velocity");
+        s->bc = *bc;
+        s->bc_set = true;
+    }
+    s->configured = true;
+    return PLB_OK;
+}
+
+int plb_step(plb_state* s, int nsteps) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!s->configured) return fail(PLB_ERR_LOGIC, "plb_step before plb_configure");
+    if (nsteps < 0) return fail(PLB_ERR_INVALID_ARGUMENT, "negative step count");
+    const double* F = s->force;
+    for (int i = 0; i < nsteps; ++i) {
+        if (s->mode == PLB_MODE_LITERAL) {
+            // PLBPD::march_in_time loop body (PLBPD.cpp:24-29)
+            const plb_boundary* bc = &s->bc;
+            if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->tau, F); }))) return rc;
+            if ((rc = timed(s, PLB_STAGE_STREAM, [&] { return lit_push(s); }))) return rc;
+            if ((rc = timed(s, PLB_STAGE_NOSLIP, [&] { return lit_noslip(s, bc); }))) return rc;
+            if ((rc = timed(s, PLB_STAGE_PERIODIC, [&] { return lit_periodic(s, bc); }))) return rc;
+            if ((rc = timed(s, PLB_STAGE_MOMENTS, [&] { return lit_moments(s); }))) return rc;
+            if ((rc = timed(s, PLB_STAGE_RESET_VELOCITY, [&] { return lit_reset_velocity(s, bc); }))) return rc;
+            if ((rc = timed(s, PLB_STAGE_RESET_DENSITY, [&] { return lit_reset_density(s, bc); }))) return rc;
+        } else {
+            if ((rc = timed(s, PLB_STAGE_FUSED, [&] { return fused_launch(s, true); }))) return rc;
+        }
+    }
+    s->collide_pending = false;
+    return PLB_OK;
+}
+
+int plb_step_graph(plb_state* s, int nsteps, int steps_per_graph) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!s->configured) return fail(PLB_ERR_LOGIC, "plb_step_graph before plb_configure");
+    if (s->mode != PLB_MODE_FUSED) return plb_step(s, nsteps);
+    if (steps_per_graph <= 0 || (steps_per_graph & 1))
+        return fail(PLB_ERR_INVALID_ARGUMENT, "steps_per_graph must be positive and even");
+    if (nsteps < 0) return fail(PLB_ERR_INVALID_ARGUMENT, "negative step count");
+    const int reps = nsteps / steps_per_graph;
+    if (reps > 0 && (!s->graph || s->graph_steps != steps_per_graph || s->graph_parity != s->cur)) {
+        drop_graph(s);
+        const bool timers = s->timers_on;
+        s->timers_on = false;
+        PLB_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        int lrc = PLB_OK;
+        for (int i = 0; i < steps_per_graph && lrc == PLB_OK; ++i) lrc = fused_launch(s, true);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s->stream, &g);
+        s->timers_on = timers;
+        if (lrc) {
+            if (g) (void)hipGraphDestroy(g);
+            return lrc;
+        }
+        if (e != hipSuccess) return fail(PLB_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            s->graph = nullptr;
+            return fail(PLB_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        s->graph_steps = steps_per_graph;
+        s->graph_parity = s->cur;  // an even number of swaps leaves the parity unchanged
+    }
+    for (int r = 0; r < reps; ++r) {
+        rc = timed(s, PLB_STAGE_FUSED, [&]() -> int {
+            PLB_HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+            return PLB_OK;
+        });
+        if (rc) return rc;
+    }
+    s->moments_valid = false;
+    const int rem = nsteps - reps * steps_per_graph;
+    return rem ? plb_step(s, rem) : PLB_OK;
+}
+
+int plb_global_moments(plb_state* s, double out[4]) {
+    int rc = enter(s);
+    if (rc) return rc;
+    if (!out) return fail(PLB_ERR_INVALID_ARGUMENT, "null output");
+    if (s->mode == PLB_MODE_LITERAL) return lit_global(s, out);
+    return fused_global(s, out);
+}
+
+int plb_synchronize(plb_state* s) {
+    int rc = enter(s);
+    if (rc) return rc;
+    PLB_HIP_TRY(hipStreamSynchronize(s->stream));
+    return PLB_OK;
+}
+
+int plb_enable_timers(plb_state* s, int on) {
+    int rc = enter(s);
+    if (rc) return rc;
+    s->timers_on = on != 0;
+    return PLB_OK;
+}
+
+int plb_stage_times(plb_state* s, double* ms_out) {
+    int rc = enter(s);
+    if (rc) return rc;
+    rc = resolve_timers(s, 0);
+    if (rc) return rc;
+    if (ms_out)
+        for (int i = 0; i < PLB_NSTAGES; ++i) ms_out[i] = s->stage_ms[i];
+    return PLB_OK;
+}
+
+}  // extern "C"

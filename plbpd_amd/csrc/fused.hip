@@ -1,0 +1,665 @@
+// Fused mode: one kernel per time step does the work of CollisionSRT,
+// Streaming (push), Boundary::apply_noslip / apply_periodicity and the
+// moments that the next collision needs (CollisionSRT.cpp:153-176,
+// Streaming.cpp:63-74, Boundary.cpp:189-281, CalcMoments.hpp:22-37), without
+// the stale seam reads of the staged step.
+//
+// Layout in HBM (structure of arrays, no halo):
+//   f[k * pstride + c],  c = (z*Y + y)*X + x,  x fastest.
+// One thread per cell; a block covers TX x-consecutive cells of TY rows, so a
+// wavefront reads 64 consecutive cells of one population (coalesced) and
+// writes them shifted by the direction's offset.  Periodic axes wrap by index
+// arithmetic; a neighbour that is a wall or solid reflects the population
+// into the cell's own reverse slot (halfway bounce-back, with the moving-wall
+// term of Boundary.cpp:213-217).
+#include <type_traits>
+#include <vector>
+
+#include "state.hpp"
+
+namespace plb {
+
+template <class T>
+struct FusedArgs {
+    uint32_t X, Y, Z;
+    uint64_t pstride;
+    uint32_t tx_shift, tx_mask;  // TX = 1 << tx_shift
+    uint32_t ntx, nty, ntiles;
+    T tau, om, omm, F0, F1, F2;
+    FusedBC<T> bc;
+};
+
+namespace {
+
+constexpr unsigned kBlock = 256;
+
+template <int Q>
+__host__ __device__ constexpr int rev_k(int k) { return rev<Q>(k); }
+
+// Workgroup i runs on XCD i % 8; give each XCD a contiguous run of tiles so
+// that the rows a cell pushes into are mostly resident in the same L2.
+__device__ inline uint32_t xcd_tile(uint32_t i, uint32_t n) {
+    const uint32_t n8 = n >> 3, r = n & 7u, j = i & 7u;
+    return j * n8 + (j < r ? j : r) + (i >> 3);
+}
+
+template <int Q, int V, int NB, bool COLLIDE>
+__global__ __launch_bounds__(kBlock) void k_fused(const typename Prec<V>::S* __restrict__ A,
+                                                  typename Prec<V>::S* __restrict__ B,
+                                                  const uint8_t* __restrict__ flags,
+                                                  const uint32_t* __restrict__ nbmask,
+                                                  FusedArgs<typename Prec<V>::T> a) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    const uint32_t t = xcd_tile(blockIdx.x, a.ntiles);
+    const uint32_t tx = t % a.ntx;
+    const uint32_t rest = t / a.ntx;
+    const uint32_t ty = rest % a.nty;
+    const uint32_t z = rest / a.nty;
+    const uint32_t x = (tx << a.tx_shift) + (threadIdx.x & a.tx_mask);
+    const uint32_t y = ty * (kBlock >> a.tx_shift) + (threadIdx.x >> a.tx_shift);
+    if (x >= a.X || y >= a.Y) return;
+    const uint64_t XY = (uint64_t)a.X * a.Y;
+    const uint64_t c = (uint64_t)z * XY + (uint64_t)y * a.X + x;
+
+    uint32_t m = 0;
+    if (NB) {
+        m = nbmask[c];
+        if (m & 1u) return;
+    } else {
+        if (flags[c]) return;
+    }
+
+    T f[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) f[k] = P::load(A[(uint64_t)k * a.pstride + c]);
+
+    T rho = (T)0;
+    if (COLLIDE || a.bc.moving) {
+        T r = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            r += f[k];
+            j0 += f[k] * (T)Lat<Q>::cx(k);
+            j1 += f[k] * (T)Lat<Q>::cy(k);
+            j2 += f[k] * (T)Lat<Q>::cz(k);
+        }
+        const T drho = r;
+        if (P::shifted) r = (T)1 + r;
+        rho = r;
+        if (COLLIDE) {
+            const T ri = (T)1 / r;
+            const T tri = a.tau / r;
+            const T u0 = j0 * ri + a.F0 * tri;
+            const T u1 = j1 * ri + a.F1 * tri;
+            const T u2 = j2 * ri + a.F2 * tri;
+            const T usq = u0 * u0 + u1 * u1 + u2 * u2;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const T cu = (k == 0) ? (T)0
+                                      : (T)Lat<Q>::cx(k) * u0 + (T)Lat<Q>::cy(k) * u1 +
+                                            (T)Lat<Q>::cz(k) * u2;
+                const T w = (T)Lat<Q>::w(k);
+                if (P::shifted) {
+                    const T geq = w * (drho + r * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
+                    f[k] = a.omm * f[k] + a.om * geq;
+                } else {
+                    const T neq = (w * r) * ((((T)1 + (T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq);
+                    f[k] = a.omm * f[k] + a.om * neq;
+                }
+            }
+        }
+    }
+
+    // neighbour coordinates with periodic wrap (walled axes never wrap for a
+    // fluid cell: the wall layer sits between it and the domain edge)
+    const uint32_t xo[3] = {x == 0 ? a.X - 1 : x - 1, x, x + 1 == a.X ? 0 : x + 1};
+    const uint32_t yo[3] = {y == 0 ? a.Y - 1 : y - 1, y, y + 1 == a.Y ? 0 : y + 1};
+    const uint32_t zo[3] = {z == 0 ? a.Z - 1 : z - 1, z, z + 1 == a.Z ? 0 : z + 1};
+    bool wx[3] = {false, false, false}, wy[3] = {false, false, false}, wz[3] = {false, false, false};
+    if (!NB) {
+        wx[0] = a.bc.wall_lo[0] && xo[0] == 0;
+        wx[2] = a.bc.wall_hi[0] && xo[2] == a.X - 1;
+        wy[0] = a.bc.wall_lo[1] && yo[0] == 0;
+        wy[2] = a.bc.wall_hi[1] && yo[2] == a.Y - 1;
+        wz[0] = a.bc.wall_lo[2] && zo[0] == 0;
+        wz[2] = a.bc.wall_hi[2] && zo[2] == a.Z - 1;
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
+        const uint64_t d = (uint64_t)zo[cz + 1] * XY + (uint64_t)yo[cy + 1] * a.X + xo[cx + 1];
+        bool solid;
+        if (NB)
+            solid = (m >> k) & 1u;
+        else
+            solid = wx[cx + 1] || wy[cy + 1] || wz[cz + 1];
+        if (solid) {
+            T v = f[k];
+            if (a.bc.moving) {
+                const int fl = flags[d];
+                const T cub = (T)cx * a.bc.vel[fl][0] + (T)cy * a.bc.vel[fl][1] + (T)cz * a.bc.vel[fl][2];
+                const T term = (T)2 * (T)Lat<Q>::w(k) * rho * cub * (T)3;
+                v = v - term;
+            }
+            B[(uint64_t)rev_k<Q>(k) * a.pstride + c] = P::store(v);
+        } else {
+            B[(uint64_t)k * a.pstride + d] = P::store(f[k]);
+        }
+    }
+}
+
+// Initial state: populations from feq(rho0, u(cell)) evaluated in double
+// (EqlbDist.hpp:26-38), stored in the variant's form.
+template <int Q, int V>
+__global__ void k_finit(typename Prec<V>::S* __restrict__ f, uint32_t X, uint32_t Y, uint32_t Z,
+                        uint64_t pstride, double rho0, double u0x, double u0y, double u0z, int kind,
+                        uint64_t seed, double amp, const double* tg) {
+    using P = Prec<V>;
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = (uint64_t)X * Y * Z;
+    if (c >= n) return;
+    const int x = (int)(c % X), y = (int)((c / X) % Y), z = (int)(c / ((uint64_t)X * Y));
+    const double u0[3] = {u0x, u0y, u0z};
+    double uc[3];
+    cell_velocity(kind, seed, amp, X, Y, Z, u0, tg, x, y, z, uc);
+    const double usq = uc[0] * uc[0] + uc[1] * uc[1] + uc[2] * uc[2];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        double v = feq_ref<Q, double>(k, rho0, uc, usq);
+        if (P::shifted) v = v - Lat<Q>::w(k);
+        typename P::S s;
+        if (V == kF16S)
+            s = (typename P::S)((float)v * 0x1.0p15f);
+        else
+            s = (typename P::S)v;
+        f[(uint64_t)k * pstride + c] = s;
+    }
+}
+
+// rho, u (double, SoA) from the populations; solids report rho_s and the wall
+// velocity, as Boundary::reset leaves wall cells (Boundary.cpp:58-89).
+template <int Q, int V>
+__global__ void k_fmoments(const typename Prec<V>::S* __restrict__ f, const uint8_t* __restrict__ flags,
+                           uint64_t n, uint64_t pstride, double rho_s, FusedBC<double> bc,
+                           double* __restrict__ rho, double* __restrict__ u) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int fl = flags[c];
+    if (fl) {
+        rho[c] = rho_s;
+        for (int i = 0; i < 3; ++i) u[i * n + c] = bc.vel[fl][i];
+        return;
+    }
+    T r = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const T fk = P::load(f[(uint64_t)k * pstride + c]);
+        r += fk;
+        j0 += fk * (T)Lat<Q>::cx(k);
+        j1 += fk * (T)Lat<Q>::cy(k);
+        j2 += fk * (T)Lat<Q>::cz(k);
+    }
+    if (P::shifted) r = (T)1 + r;
+    const T ri = (T)1 / r;
+    rho[c] = (double)r;
+    u[c] = (double)(j0 * ri);
+    u[n + c] = (double)(j1 * ri);
+    u[2 * n + c] = (double)(j2 * ri);
+}
+
+// Populations: device SoA <-> host AoS, plain f in H (shift added back).
+template <int Q, int V, class H>
+__global__ void k_fpop(typename Prec<V>::S* __restrict__ f, H* __restrict__ host, uint32_t X,
+                       uint32_t Y, uint32_t Z, uint64_t pstride, int reference_layout, int upload) {
+    using P = Prec<V>;
+    const uint64_t n = (uint64_t)X * Y * Z;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * Q) return;
+    const int k = (int)(t % Q);
+    const uint64_t c = t / Q;
+    uint64_t h = t;
+    if (reference_layout) {
+        const uint64_t x = c % X, y = (c / X) % Y, z = c / ((uint64_t)X * Y);
+        h = ((((z + 1) * (Y + 2)) + y + 1) * (X + 2) + x + 1) * Q + k;
+    }
+    if (upload) {
+        double v = (double)host[h];
+        if (P::shifted) v = v - Lat<Q>::w(k);
+        typename P::S s;
+        if (V == kF16S)
+            s = (typename P::S)((float)v * 0x1.0p15f);
+        else
+            s = (typename P::S)v;
+        f[(uint64_t)k * pstride + c] = s;
+    } else {
+        double v = (double)P::load(f[(uint64_t)k * pstride + c]);
+        if (P::shifted) v = v + Lat<Q>::w(k);
+        host[h] = (H)v;
+    }
+}
+
+// moments (double SoA) -> host AoS in H
+template <class H>
+__global__ void k_fmom_out(const double* __restrict__ src, H* __restrict__ host, uint32_t X,
+                           uint32_t Y, uint32_t Z, int vlen, int reference_layout) {
+    const uint64_t n = (uint64_t)X * Y * Z;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * vlen) return;
+    const int v = (int)(t % vlen);
+    const uint64_t c = t / vlen;
+    uint64_t h = t;
+    if (reference_layout) {
+        const uint64_t x = c % X, y = (c / X) % Y, z = c / ((uint64_t)X * Y);
+        h = ((((z + 1) * (Y + 2)) + y + 1) * (X + 2) + x + 1) * vlen + v;
+    }
+    host[h] = (H)src[v * n + c];
+}
+
+template <class S>
+__global__ void k_fraw(const S* __restrict__ f, S* __restrict__ out, uint64_t n, uint64_t pstride, int q) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * q) return;
+    const int k = (int)(t % q);
+    const uint64_t c = t / q;
+    out[t] = f[(uint64_t)k * pstride + c];
+}
+
+// Neighbour-solid mask for domains with interior solids: bit 0 = the cell
+// itself is not fluid, bit k = neighbour x + c_k is not fluid.
+template <int Q>
+__global__ void k_nbmask(const uint8_t* __restrict__ flags, uint32_t* __restrict__ mask, uint32_t X,
+                         uint32_t Y, uint32_t Z) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = (uint64_t)X * Y * Z;
+    if (c >= n) return;
+    const uint32_t x = c % X, y = (c / X) % Y, z = c / ((uint64_t)X * Y);
+    uint32_t m = flags[c] ? 1u : 0u;
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+        const uint32_t xd = (x + X + Lat<Q>::cx(k)) % X, yd = (y + Y + Lat<Q>::cy(k)) % Y,
+                       zd = (z + Z + Lat<Q>::cz(k)) % Z;
+        if (flags[((uint64_t)zd * Y + yd) * X + xd]) m |= 1u << k;
+    }
+    mask[c] = m;
+}
+
+template <int Q, int V>
+__global__ void k_fglobal_partial(const typename Prec<V>::S* __restrict__ f,
+                                  const uint8_t* __restrict__ flags, uint64_t n, uint64_t pstride,
+                                  double* __restrict__ partial) {
+    using P = Prec<V>;
+    __shared__ double sh[4][kBlock];
+    double acc[4] = {0, 0, 0, 0};
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        if (flags[c]) continue;
+        double r = 0, j0 = 0, j1 = 0, j2 = 0;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const double fk = (double)P::load(f[(uint64_t)k * pstride + c]);
+            r += fk;
+            j0 += fk * Lat<Q>::cx(k);
+            j1 += fk * Lat<Q>::cy(k);
+            j2 += fk * Lat<Q>::cz(k);
+        }
+        if (P::shifted) r = 1.0 + r;
+        acc[0] += r;
+        acc[1] += j0;
+        acc[2] += j1;
+        acc[3] += j2;
+    }
+    for (int d = 0; d < 4; ++d) sh[d][threadIdx.x] = acc[d];
+    __syncthreads();
+    for (unsigned s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int d = 0; d < 4; ++d) sh[d][threadIdx.x] += sh[d][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 4; ++d) partial[4 * blockIdx.x + d] = sh[d][0];
+}
+
+__global__ void k_fglobal_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a[4] = {0, 0, 0, 0};
+    for (int b = 0; b < nblocks; ++b)
+        for (int d = 0; d < 4; ++d) a[d] += partial[4 * b + d];
+    for (int d = 0; d < 4; ++d) out[d] = a[d];
+}
+
+template <class Fn>
+int with_qv(int q, int v, Fn&& fn) {
+    auto vsel = [&](auto qc) {
+        switch (v) {
+            case kF64: return fn(qc, std::integral_constant<int, kF64>());
+            case kF32: return fn(qc, std::integral_constant<int, kF32>());
+            case kF64A32: return fn(qc, std::integral_constant<int, kF64A32>());
+            case kF32S: return fn(qc, std::integral_constant<int, kF32S>());
+            case kF16S: return fn(qc, std::integral_constant<int, kF16S>());
+            default: return fail(PLB_ERR_INVALID_ARGUMENT, "unknown storage variant");
+        }
+    };
+    switch (q) {
+        case 9: return vsel(std::integral_constant<int, 9>());
+        case 19: return vsel(std::integral_constant<int, 19>());
+        case 27: return vsel(std::integral_constant<int, 27>());
+        default: return fail(PLB_ERR_LOGIC, "Model with q=" + std::to_string(q) + " not yet implemented");
+    }
+}
+
+int ensure_scratch(plb_state* s, size_t bytes) {
+    if (s->scratch_bytes >= bytes) return PLB_OK;
+    if (s->scratch) PLB_HIP_TRY(hipFree(s->scratch));
+    s->scratch = nullptr;
+    s->scratch_bytes = 0;
+    PLB_HIP_TRY(hipMalloc(&s->scratch, bytes));
+    s->scratch_bytes = bytes;
+    return PLB_OK;
+}
+
+bool lattice_moves(int q, int axis) {
+    if (axis == 1 && q == 9) return false;  // D2Q9 lives in the x-z plane
+    return true;
+}
+
+}  // namespace
+
+int check_fused_bc(int q, int X, int Y, int Z, const plb_boundary* bc) {
+    const int dims[3] = {X, Y, Z};
+    for (int a = 0; a < 3; ++a) {
+        if (!lattice_moves(q, a)) continue;
+        const int hi = bc->type[2 * a], lo = bc->type[2 * a + 1];
+        const bool per = hi == kPeriodic && lo == kPeriodic;
+        const bool wall = hi == kNoslip && lo == kNoslip;
+        if (!per && !wall)
+            return fail(PLB_ERR_LOGIC,
+                        "fused mode needs each axis periodic on both faces or no-slip on both faces");
+        if (wall && dims[a] < 3) return fail(PLB_ERR_INVALID_ARGUMENT, "walled axis shorter than 3 cells");
+    }
+    for (int f = 0; f < 6; ++f)
+        if (bc->type[f] == kNoslip && !bc->has_vel[f])
+            return fail(PLB_ERR_OUT_OF_RANGE, "no-slip face without a prescribed velocity");
+    return PLB_OK;
+}
+
+template <class T>
+void make_fused_bc(const plb_boundary& bc, int q, FusedBC<T>* out) {
+    *out = FusedBC<T>();
+    for (int a = 0; a < 3; ++a) {
+        const bool moves = lattice_moves(q, a);
+        out->periodic[a] = moves && bc.type[2 * a] == kPeriodic && bc.type[2 * a + 1] == kPeriodic;
+        out->wall_hi[a] = moves && bc.type[2 * a] == kNoslip;
+        out->wall_lo[a] = moves && bc.type[2 * a + 1] == kNoslip;
+    }
+    out->moving = 0;
+    for (int i = 0; i < 8; ++i)
+        for (int d = 0; d < 3; ++d) out->vel[i][d] = (T)0;
+    for (int f = 0; f < 6; ++f) {
+        if (bc.type[f] != kNoslip) continue;
+        for (int d = 0; d < 3; ++d) {
+            out->vel[1 + f][d] = (T)bc.vel[f][d];
+            if (bc.vel[f][d] != 0.0) out->moving = 1;
+        }
+    }
+}
+template void make_fused_bc<double>(const plb_boundary&, int, FusedBC<double>*);
+template void make_fused_bc<float>(const plb_boundary&, int, FusedBC<float>*);
+
+int fused_build_flags(plb_state* s, const plb_boundary* bc, const uint8_t* mask) {
+    int rc = check_fused_bc(s->q, s->X, s->Y, s->Z, bc);
+    if (rc) return rc;
+    const uint64_t n = s->ncell;
+    std::vector<uint8_t> fl(n);
+    bool any_solid = false;
+    for (uint32_t z = 0; z < (uint32_t)s->Z; ++z)
+        for (uint32_t y = 0; y < (uint32_t)s->Y; ++y)
+            for (uint32_t x = 0; x < (uint32_t)s->X; ++x) {
+                const uint64_t c = ((uint64_t)z * s->Y + y) * s->X + x;
+                int f8 = (mask && mask[c]) ? kFlagSolid : kFlagFluid;
+                if (f8) any_solid = true;
+                const bool on[6] = {x == (uint32_t)s->X - 1, x == 0, y == (uint32_t)s->Y - 1,
+                                    y == 0, z == (uint32_t)s->Z - 1, z == 0};
+                for (int f = 0; f < 6; ++f)
+                    if (bc->type[f] == kNoslip && on[f] && lattice_moves(s->q, f / 2)) f8 = 1 + f;
+                fl[c] = (uint8_t)f8;
+            }
+    PLB_HIP_TRY(hipMemcpyAsync(s->flags, fl.data(), n, hipMemcpyHostToDevice, s->stream));
+    if (any_solid) {
+        if (!s->nbmask) PLB_HIP_TRY(hipMalloc(&s->nbmask, n * sizeof(uint32_t)));
+        rc = with_qv(s->q, kF64, [&](auto qc, auto) {
+            constexpr int Q = decltype(qc)::value;
+            hipLaunchKernelGGL((k_nbmask<Q>), dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s->stream,
+                               (const uint8_t*)s->flags, s->nbmask, (uint32_t)s->X, (uint32_t)s->Y,
+                               (uint32_t)s->Z);
+            PLB_HIP_TRY(hipGetLastError());
+            return PLB_OK;
+        });
+        if (rc) return rc;
+    }
+    PLB_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->has_solids = any_solid;
+    s->bc = *bc;
+    s->bc_set = true;
+    return PLB_OK;
+}
+
+int fused_initialize(plb_state* s, double rho0, const double u0[3], const plb_init_spec* init,
+                     const double* tg_dev) {
+    const int kind = init ? init->kind : PLB_INIT_UNIFORM;
+    const uint64_t seed = init ? init->seed : 0;
+    const double amp = init ? init->amp : 0.0;
+    return with_qv(s->q, s->variant, [&](auto qc, auto vc) {
+        constexpr int Q = decltype(qc)::value;
+        constexpr int V = decltype(vc)::value;
+        using S = typename Prec<V>::S;
+        hipLaunchKernelGGL((k_finit<Q, V>), dim3(blocks_for(s->ncell, kBlock)), dim3(kBlock), 0,
+                           s->stream, (S*)s->buf[s->cur], (uint32_t)s->X, (uint32_t)s->Y,
+                           (uint32_t)s->Z, (uint64_t)s->pstride, rho0, u0[0], u0[1], u0[2], kind, seed,
+                           amp, tg_dev);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+}
+
+int fused_launch(plb_state* s, bool collide) {
+    if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
+    unsigned tx = 64;
+    if (s->X % 256 == 0)
+        tx = 256;
+    else if (s->X % 128 == 0)
+        tx = 128;
+    const unsigned ty = kBlock / tx;
+    const uint32_t ntx = (uint32_t)((s->X + tx - 1) / tx);
+    const uint32_t nty = (uint32_t)((s->Y + ty - 1) / ty);
+    const uint64_t ntiles = (uint64_t)ntx * nty * s->Z;
+    if (ntiles >= (1ull << 31)) return fail(PLB_ERR_INVALID_ARGUMENT, "domain too large for one launch");
+    const int rc = with_qv(s->q, s->variant, [&](auto qc, auto vc) {
+        constexpr int Q = decltype(qc)::value;
+        constexpr int V = decltype(vc)::value;
+        using P = Prec<V>;
+        using S = typename P::S;
+        using T = typename P::T;
+        FusedArgs<T> a;
+        a.X = s->X;
+        a.Y = s->Y;
+        a.Z = s->Z;
+        a.pstride = s->pstride;
+        a.tx_shift = (uint32_t)__builtin_ctz(tx);
+        a.tx_mask = tx - 1;
+        a.ntx = ntx;
+        a.nty = nty;
+        a.ntiles = (uint32_t)ntiles;
+        a.tau = (T)s->tau;
+        if (std::is_same<T, float>::value)
+            a.om = (T)(1. / (double)a.tau);
+        else
+            a.om = (T)(1.0 / s->tau);
+        a.omm = (T)1 - a.om;
+        a.F0 = (T)s->force[0];
+        a.F1 = (T)s->force[1];
+        a.F2 = (T)s->force[2];
+        make_fused_bc<T>(s->bc, s->q, &a.bc);
+        const S* A = (const S*)s->buf[s->cur];
+        S* B = (S*)s->buf[s->cur ^ 1];
+        const dim3 grid((unsigned)ntiles), block(kBlock);
+        if (s->has_solids) {
+            if (collide)
+                hipLaunchKernelGGL((k_fused<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+            else
+                hipLaunchKernelGGL((k_fused<Q, V, 1, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+        } else {
+            if (collide)
+                hipLaunchKernelGGL((k_fused<Q, V, 0, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+            else
+                hipLaunchKernelGGL((k_fused<Q, V, 0, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+        }
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+    if (rc == PLB_OK) {
+        s->cur ^= 1;
+        s->moments_valid = false;
+    }
+    return rc;
+}
+
+int fused_moments(plb_state* s) {
+    if (!s->bc_set) return fail(PLB_ERR_LOGIC, "moments before the boundary was set");
+    FusedBC<double> bc;
+    make_fused_bc<double>(s->bc, s->q, &bc);
+    const int rc = with_qv(s->q, s->variant, [&](auto qc, auto vc) {
+        constexpr int Q = decltype(qc)::value;
+        constexpr int V = decltype(vc)::value;
+        using S = typename Prec<V>::S;
+        hipLaunchKernelGGL((k_fmoments<Q, V>), dim3(blocks_for(s->ncell, kBlock)), dim3(kBlock), 0,
+                           s->stream, (const S*)s->buf[s->cur], (const uint8_t*)s->flags,
+                           (uint64_t)s->ncell, (uint64_t)s->pstride, s->bc.rho_s, bc, (double*)s->rho,
+                           (double*)s->u);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+    if (rc == PLB_OK) s->moments_valid = true;
+    return rc;
+}
+
+int fused_transfer(plb_state* s, int field, int layout, int dtype, void* host, bool upload) {
+    const int ref = layout == PLB_LAYOUT_REFERENCE;
+    const uint64_t n = s->ncell;
+    if (field == PLB_FIELD_NTMP) return fail(PLB_ERR_LOGIC, "ntmp is not kept in fused mode");
+    if (field != PLB_FIELD_N && field != PLB_FIELD_RHO && field != PLB_FIELD_U)
+        return fail(PLB_ERR_INVALID_ARGUMENT, "unknown field");
+    if (upload && field != PLB_FIELD_N)
+        return fail(PLB_ERR_LOGIC, "rho and u are derived from the populations in fused mode");
+    const int vlen = field == PLB_FIELD_N ? s->q : (field == PLB_FIELD_RHO ? 1 : 3);
+    const uint64_t hcells = ref ? (uint64_t)(s->X + 2) * (s->Y + 2) * (s->Z + 2) : n;
+    const size_t esz = dtype == PLB_DTYPE_F64 ? 8 : 4;
+    const size_t hbytes = hcells * vlen * esz;
+    if (field != PLB_FIELD_N && !s->moments_valid) {
+        const int rc = fused_moments(s);
+        if (rc) return rc;
+    }
+    void* stage = nullptr;
+    PLB_HIP_TRY(hipMalloc(&stage, hbytes));
+    int rc = PLB_OK;
+    // a host array in reference layout keeps its halo: stage it both ways
+    if (upload || ref) {
+        hipError_t e = hipMemcpyAsync(stage, host, hbytes, hipMemcpyHostToDevice, s->stream);
+        if (e != hipSuccess) rc = fail(PLB_ERR_HIP, hipGetErrorString(e));
+    }
+    if (rc == PLB_OK) {
+        if (field == PLB_FIELD_N) {
+            rc = with_qv(s->q, s->variant, [&](auto qc, auto vc) {
+                constexpr int Q = decltype(qc)::value;
+                constexpr int V = decltype(vc)::value;
+                using S = typename Prec<V>::S;
+                const unsigned nb = blocks_for(n * Q, kBlock);
+                if (dtype == PLB_DTYPE_F64)
+                    hipLaunchKernelGGL((k_fpop<Q, V, double>), dim3(nb), dim3(kBlock), 0, s->stream,
+                                       (S*)s->buf[s->cur], (double*)stage, (uint32_t)s->X, (uint32_t)s->Y,
+                                       (uint32_t)s->Z, (uint64_t)s->pstride, ref, upload ? 1 : 0);
+                else
+                    hipLaunchKernelGGL((k_fpop<Q, V, float>), dim3(nb), dim3(kBlock), 0, s->stream,
+                                       (S*)s->buf[s->cur], (float*)stage, (uint32_t)s->X, (uint32_t)s->Y,
+                                       (uint32_t)s->Z, (uint64_t)s->pstride, ref, upload ? 1 : 0);
+                PLB_HIP_TRY(hipGetLastError());
+                return PLB_OK;
+            });
+        } else {
+            const double* src = field == PLB_FIELD_RHO ? (const double*)s->rho : (const double*)s->u;
+            const unsigned nb = blocks_for(n * vlen, kBlock);
+            if (dtype == PLB_DTYPE_F64)
+                hipLaunchKernelGGL((k_fmom_out<double>), dim3(nb), dim3(kBlock), 0, s->stream, src,
+                                   (double*)stage, (uint32_t)s->X, (uint32_t)s->Y, (uint32_t)s->Z, vlen, ref);
+            else
+                hipLaunchKernelGGL((k_fmom_out<float>), dim3(nb), dim3(kBlock), 0, s->stream, src,
+                                   (float*)stage, (uint32_t)s->X, (uint32_t)s->Y, (uint32_t)s->Z, vlen, ref);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = fail(PLB_ERR_HIP, hipGetErrorString(e));
+        }
+    }
+    if (rc == PLB_OK && !upload) {
+        hipError_t e = hipMemcpyAsync(host, stage, hbytes, hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) rc = fail(PLB_ERR_HIP, hipGetErrorString(e));
+    }
+    hipError_t e = hipStreamSynchronize(s->stream);
+    (void)hipFree(stage);
+    if (rc == PLB_OK && e != hipSuccess) rc = fail(PLB_ERR_HIP, hipGetErrorString(e));
+    if (rc == PLB_OK && upload) s->moments_valid = false;
+    return rc;
+}
+
+int fused_download_raw(plb_state* s, void* host, size_t bytes) {
+    const size_t need = s->ncell * s->q * s->sbytes;
+    if (bytes < need) return fail(PLB_ERR_INVALID_ARGUMENT, "raw buffer too small");
+    void* stage = nullptr;
+    PLB_HIP_TRY(hipMalloc(&stage, need));
+    const unsigned nb = blocks_for(s->ncell * s->q, kBlock);
+    if (s->sbytes == 8)
+        hipLaunchKernelGGL((k_fraw<uint64_t>), dim3(nb), dim3(kBlock), 0, s->stream,
+                           (const uint64_t*)s->buf[s->cur], (uint64_t*)stage, (uint64_t)s->ncell,
+                           (uint64_t)s->pstride, s->q);
+    else if (s->sbytes == 4)
+        hipLaunchKernelGGL((k_fraw<uint32_t>), dim3(nb), dim3(kBlock), 0, s->stream,
+                           (const uint32_t*)s->buf[s->cur], (uint32_t*)stage, (uint64_t)s->ncell,
+                           (uint64_t)s->pstride, s->q);
+    else
+        hipLaunchKernelGGL((k_fraw<uint16_t>), dim3(nb), dim3(kBlock), 0, s->stream,
+                           (const uint16_t*)s->buf[s->cur], (uint16_t*)stage, (uint64_t)s->ncell,
+                           (uint64_t)s->pstride, s->q);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, stage, need, hipMemcpyDeviceToHost, s->stream);
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(PLB_ERR_HIP, hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(PLB_ERR_HIP, hipGetErrorString(e2));
+    return PLB_OK;
+}
+
+int fused_global(plb_state* s, double out[4]) {
+    if (!s->bc_set) return fail(PLB_ERR_LOGIC, "global moments before the boundary was set");
+    const int nb = 1024;
+    int rc = ensure_scratch(s, (size_t)(4 * nb + 4) * sizeof(double));
+    if (rc) return rc;
+    double* partial = (double*)s->scratch;
+    rc = with_qv(s->q, s->variant, [&](auto qc, auto vc) {
+        constexpr int Q = decltype(qc)::value;
+        constexpr int V = decltype(vc)::value;
+        using S = typename Prec<V>::S;
+        hipLaunchKernelGGL((k_fglobal_partial<Q, V>), dim3(nb), dim3(kBlock), 0, s->stream,
+                           (const S*)s->buf[s->cur], (const uint8_t*)s->flags, (uint64_t)s->ncell,
+                           (uint64_t)s->pstride, partial);
+        PLB_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_fglobal_final, dim3(1), dim3(64), 0, s->stream, partial, nb, partial + 4 * nb);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+    if (rc) return rc;
+    PLB_HIP_TRY(hipMemcpyAsync(out, partial + 4 * nb, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PLB_HIP_TRY(hipStreamSynchronize(s->stream));
+    return PLB_OK;
+}
+
+}  // namespace plb

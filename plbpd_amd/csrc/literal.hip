@@ -1,0 +1,576 @@
+// Literal mode: the reference's staged step (PLBPD.cpp:23-30), one kernel per
+// reference operator, on halo'd structure-of-arrays buffers:
+//   n[k * ncell + c],  c = (z*(Y+2) + y)*(X+2) + x,  x in [0, X+1] etc.
+// R = double (FP64 contract) or float (the reference's own rounding, with its
+// float/double promotions).  Compiled with -ffp-contract=off: every
+// expression below is evaluated in the order written, as in the reference.
+#include <type_traits>
+
+#include "state.hpp"
+
+namespace plb {
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+struct LitDims {
+    int X, Y, Z;
+    int nx, ny;      // X+2, Y+2
+    uint64_t ncell;  // (X+2)(Y+2)(Z+2)
+    __host__ __device__ uint64_t cell(int z, int y, int x) const {
+        return ((uint64_t)z * ny + y) * nx + x;
+    }
+};
+
+LitDims dims_of(const plb_state* s) {
+    LitDims d;
+    d.X = s->X;
+    d.Y = s->Y;
+    d.Z = s->Z;
+    d.nx = s->X + 2;
+    d.ny = s->Y + 2;
+    d.ncell = s->ncell;
+    return d;
+}
+
+template <class Fn>
+int with_q(int q, Fn&& fn) {
+    switch (q) {
+        case 9: return fn(std::integral_constant<int, 9>());
+        case 19: return fn(std::integral_constant<int, 19>());
+        case 27: return fn(std::integral_constant<int, 27>());
+        default: return fail(PLB_ERR_LOGIC, "Model with q=" + std::to_string(q) + " not yet implemented");
+    }
+}
+
+template <class Fn>
+int with_qr(const plb_state* s, Fn&& fn) {
+    return with_q(s->q, [&](auto qc) {
+        if (s->variant == kF64) return fn(qc, double());
+        return fn(qc, float());
+    });
+}
+
+// ---------------------------------------------------------------- kernels --
+// Domain::initialize (Domain.cpp:46-64): every cell, halo included, gets
+// rho0, u and feq(rho0, u); ntmp is zero (SimData.cpp:53-56).
+template <int Q, class R>
+__global__ void k_init(LitDims g, R* n, R* ntmp, R* rho, R* u, double rho0, double u0x,
+                       double u0y, double u0z, int kind, uint64_t seed, double amp,
+                       const double* tg) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.ncell) return;
+    const int x = (int)(c % g.nx);
+    const int y = (int)((c / g.nx) % g.ny);
+    const int z = (int)(c / ((uint64_t)g.nx * g.ny));
+    const int xi = (x - 1 + g.X) % g.X, yi = (y - 1 + g.Y) % g.Y, zi = (z - 1 + g.Z) % g.Z;
+    const double u0[3] = {u0x, u0y, u0z};
+    double uc[3];
+    cell_velocity(kind, seed, amp, g.X, g.Y, g.Z, u0, tg, xi, yi, zi, uc);
+    const R ur[3] = {(R)uc[0], (R)uc[1], (R)uc[2]};
+    const R r0 = (R)rho0;
+    rho[c] = r0;
+    for (int i = 0; i < 3; ++i) u[i * g.ncell + c] = ur[i];
+    const R usq = ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        n[k * g.ncell + c] = feq_ref<Q, R>(k, r0, ur, usq);
+        ntmp[k * g.ncell + c] = (R)0;
+    }
+}
+
+// CollisionSRT::_collision_kernel (CollisionSRT.cpp:153-176) with
+// _get_updated_u (72-81) and _get_cu (84-91), in place, every interior cell.
+template <int Q, class R>
+__global__ void k_collide(LitDims g, R* __restrict__ n, const R* __restrict__ rho,
+                          const R* __restrict__ u, R tau, R om, R omm, R F0, R F1, R F2) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int y = blockIdx.y + 1, z = blockIdx.z + 1;
+    if (x > g.X) return;
+    const uint64_t c = g.cell(z, y, x);
+    const R rl = rho[c];
+    const R tri = tau / rl;
+    const R uu[3] = {u[c] + F0 * tri, u[g.ncell + c] + F1 * tri, u[2 * g.ncell + c] + F2 * tri};
+    const R usq = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const R cu = (k == 0) ? (R)0
+                              : (R)Lat<Q>::cx(k) * uu[0] + (R)Lat<Q>::cy(k) * uu[1] +
+                                    (R)Lat<Q>::cz(k) * uu[2];
+        R* p = n + k * g.ncell + c;
+        if (std::is_same<R, float>::value) {
+            const float wr = (float)Lat<Q>::w(k) * (float)rl;
+            const double neq = (double)wr * (1.0 + 3.0 * (double)cu + 4.5 * (double)cu * (double)cu -
+                                             1.5 * (double)usq);
+            const float a = (float)omm * (float)*p;
+            *p = (R)((double)a + (double)om * neq);
+        } else {
+            const double neq = (Lat<Q>::w(k) * (double)rl) *
+                               (1.0 + 3.0 * (double)cu + 4.5 * (double)cu * (double)cu -
+                                1.5 * (double)usq);
+            *p = (R)((double)omm * (double)*p + (double)om * neq);
+        }
+    }
+}
+
+// Streaming::_push_kernel (Streaming.cpp:63-74): ntmp(x + c_k, k) = n(x, k).
+template <int Q, class R>
+__global__ void k_push(LitDims g, const R* __restrict__ n, R* __restrict__ t) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int y = blockIdx.y + 1, z = blockIdx.z + 1;
+    if (x > g.X) return;
+    const uint64_t c = g.cell(z, y, x);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int64_t off = (int64_t)Lat<Q>::cx(k) + (int64_t)Lat<Q>::cy(k) * g.nx +
+                            (int64_t)Lat<Q>::cz(k) * g.nx * g.ny;
+        t[k * g.ncell + (uint64_t)((int64_t)c + off)] = n[k * g.ncell + c];
+    }
+}
+
+struct FaceBox {
+    int lo[3], ex[3];  // first cell and extent per axis (halo'd coordinates)
+    __host__ __device__ uint64_t count() const { return (uint64_t)ex[0] * ex[1] * ex[2]; }
+};
+
+// Boundary::_get_boundary_extent (Boundary.cpp:113-148)
+FaceBox face_box(int face, int X, int Y, int Z) {
+    FaceBox b;
+    b.lo[0] = 1; b.ex[0] = X;
+    b.lo[1] = 1; b.ex[1] = Y;
+    b.lo[2] = 1; b.ex[2] = Z;
+    const int axis = face / 2;
+    const int hi = (face % 2) == 0;  // east, north, up are the high faces
+    const int dim = axis == 0 ? X : (axis == 1 ? Y : Z);
+    b.lo[axis] = hi ? dim : 1;
+    b.ex[axis] = 1;
+    return b;
+}
+
+// Boundary::_apply_noslip (Boundary.cpp:189-221) for one face, as the serial
+// z, y, x, kp loop computes it.  A destination (y, kp) of face cell c is
+// written once; its value reads n(c, k = rev kp).  When y is itself in the
+// face and comes before c in the loop order, op (y, rev kp) has already
+// replaced n(c, k) by n(y, kp) - t(c): the value is then
+// (n(y, kp) - t(c)) - t(y).  Otherwise n(c, k) is still the value from before
+// the face pass.  Every read below is of the values from before the pass, so
+// the gather kernel writes to a staging buffer and a second kernel scatters.
+template <int Q, class R>
+__global__ void k_noslip_gather(LitDims g, FaceBox b, const R* __restrict__ n,
+                                const R* __restrict__ rho, R ub0, R ub1, R ub2, R cs2inv,
+                                R* __restrict__ stage, uint64_t total) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const uint64_t fi = t / (Q - 1);
+    const int kp = 1 + (int)(t % (Q - 1));
+    const int x = b.lo[0] + (int)(fi % b.ex[0]);
+    const int y = b.lo[1] + (int)((fi / b.ex[0]) % b.ex[1]);
+    const int z = b.lo[2] + (int)(fi / ((uint64_t)b.ex[0] * b.ex[1]));
+    int cxp = 0, cyp = 0, czp = 0, k = 0;
+#pragma unroll
+    for (int j = 1; j < Q; ++j)
+        if (j == kp) {
+            cxp = Lat<Q>::cx(j);
+            cyp = Lat<Q>::cy(j);
+            czp = Lat<Q>::cz(j);
+            k = rev<Q>(j);
+        }
+    const int yx = x + cxp, yy = y + cyp, yz = z + czp;
+    const uint64_t cc = g.cell(z, y, x), cy = g.cell(yz, yy, yx);
+    const R wk = (R)Lat<Q>::w(k);  // w_k == w_kp
+    // c_k = -c_kp
+    const R cu_k = (R)(-cxp) * ub0 + (R)(-cyp) * ub1 + (R)(-czp) * ub2;
+    const R t_y = (R)2 * wk * rho[cy] * cu_k * cs2inv;
+    const bool in_face = yx >= b.lo[0] && yx < b.lo[0] + b.ex[0] && yy >= b.lo[1] &&
+                         yy < b.lo[1] + b.ex[1] && yz >= b.lo[2] && yz < b.lo[2] + b.ex[2];
+    R v;
+    if (in_face && cy < cc) {
+        const R cu_kp = (R)cxp * ub0 + (R)cyp * ub1 + (R)czp * ub2;
+        const R t_c = (R)2 * wk * rho[cc] * cu_kp * cs2inv;
+        v = (n[kp * g.ncell + cy] - t_c) - t_y;
+    } else {
+        v = n[k * g.ncell + cc] - t_y;
+    }
+    stage[t] = v;
+}
+
+template <int Q, class R>
+__global__ void k_noslip_scatter(LitDims g, FaceBox b, R* __restrict__ n,
+                                 const R* __restrict__ stage, uint64_t total) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const uint64_t fi = t / (Q - 1);
+    const int kp = 1 + (int)(t % (Q - 1));
+    const int x = b.lo[0] + (int)(fi % b.ex[0]);
+    const int y = b.lo[1] + (int)((fi / b.ex[0]) % b.ex[1]);
+    const int z = b.lo[2] + (int)(fi / ((uint64_t)b.ex[0] * b.ex[1]));
+    int cxp = 0, cyp = 0, czp = 0;
+#pragma unroll
+    for (int j = 1; j < Q; ++j)
+        if (j == kp) {
+            cxp = Lat<Q>::cx(j);
+            cyp = Lat<Q>::cy(j);
+            czp = Lat<Q>::cz(j);
+        }
+    n[kp * g.ncell + g.cell(z + czp, y + cyp, x + cxp)] = stage[t];
+}
+
+// Boundary::_apply_periodicity_east_west (Boundary.cpp:234-251): for
+// z in 1..Z, y in 1..Y: n(1, k) = n(X+1, k) for c_kx > 0, n(X, k) = n(0, k)
+// for c_kx < 0.  axis 1 is _north_south (264-281) with x in 1..X.
+template <int Q, class R>
+__global__ void k_periodic(LitDims g, int axis, R* __restrict__ n) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int other = axis == 0 ? g.Y : g.X;
+    const uint64_t total = (uint64_t)g.Z * other * (Q - 1);
+    if (t >= total) return;
+    const int k = 1 + (int)(t % (Q - 1));
+    const int a = 1 + (int)((t / (Q - 1)) % other);
+    const int z = 1 + (int)(t / ((uint64_t)(Q - 1) * other));
+    int ck = 0;
+#pragma unroll
+    for (int j = 1; j < Q; ++j)
+        if (j == k) ck = axis == 0 ? Lat<Q>::cx(j) : Lat<Q>::cy(j);
+    const int hi = axis == 0 ? g.X : g.Y;
+    uint64_t dst, src;
+    if (ck > 0) {
+        dst = axis == 0 ? g.cell(z, a, 1) : g.cell(z, 1, a);
+        src = axis == 0 ? g.cell(z, a, hi + 1) : g.cell(z, hi + 1, a);
+    } else if (ck < 0) {
+        dst = axis == 0 ? g.cell(z, a, hi) : g.cell(z, hi, a);
+        src = axis == 0 ? g.cell(z, a, 0) : g.cell(z, 0, a);
+    } else {
+        return;
+    }
+    n[k * g.ncell + dst] = n[k * g.ncell + src];
+}
+
+// CalcMoments::_get_local_moments (CalcMoments.hpp:22-37) on interior cells.
+template <int Q, class R>
+__global__ void k_moments(LitDims g, const R* __restrict__ n, R* __restrict__ rho,
+                          R* __restrict__ u) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int y = blockIdx.y + 1, z = blockIdx.z + 1;
+    if (x > g.X) return;
+    const uint64_t c = g.cell(z, y, x);
+    R rl = (R)0, u0 = (R)0, u1 = (R)0, u2 = (R)0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const R nk = n[k * g.ncell + c];
+        rl += nk;
+        u0 += nk * (R)Lat<Q>::cx(k);
+        u1 += nk * (R)Lat<Q>::cy(k);
+        u2 += nk * (R)Lat<Q>::cz(k);
+    }
+    const R ri = (R)1 / rl;
+    rho[c] = rl;
+    u[c] = u0 * ri;
+    u[g.ncell + c] = u1 * ri;
+    u[2 * g.ncell + c] = u2 * ri;
+}
+
+// Boundary::_set_velocity / _set_density (Boundary.cpp:150-187) on one face.
+template <class R>
+__global__ void k_set_face(LitDims g, FaceBox b, R* __restrict__ dst, int vlen, R v0, R v1, R v2) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count()) return;
+    const int x = b.lo[0] + (int)(t % b.ex[0]);
+    const int y = b.lo[1] + (int)((t / b.ex[0]) % b.ex[1]);
+    const int z = b.lo[2] + (int)(t / ((uint64_t)b.ex[0] * b.ex[1]));
+    const uint64_t c = g.cell(z, y, x);
+    dst[c] = v0;
+    if (vlen == 3) {
+        dst[g.ncell + c] = v1;
+        dst[2 * g.ncell + c] = v2;
+    }
+}
+
+// Host AoS <-> device SoA with dtype conversion.  compact: host cells are
+// interior only (X*Y*Z); otherwise the full halo'd array.
+template <class H, class R>
+__global__ void k_transfer(LitDims g, H* __restrict__ host, R* __restrict__ dev, int vlen,
+                           int compact, int upload, uint64_t total) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int v = (int)(t % vlen);
+    const uint64_t hc = t / vlen;
+    uint64_t c = hc;
+    if (compact) {
+        const int x = (int)(hc % g.X), y = (int)((hc / g.X) % g.Y), z = (int)(hc / ((uint64_t)g.X * g.Y));
+        c = g.cell(z + 1, y + 1, x + 1);
+    }
+    if (upload)
+        dev[v * g.ncell + c] = (R)host[t];
+    else
+        host[t] = (H)dev[v * g.ncell + c];
+}
+
+// FP64 sums over interior cells of rho and rho*u (two deterministic passes).
+template <class R>
+__global__ void k_global_partial(LitDims g, const R* __restrict__ rho, const R* __restrict__ u,
+                                 double* __restrict__ partial) {
+    __shared__ double sh[4][kThreads];
+    double acc[4] = {0, 0, 0, 0};
+    const uint64_t total = (uint64_t)g.X * g.Y * g.Z;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % g.X) + 1, y = (int)((i / g.X) % g.Y) + 1,
+                  z = (int)(i / ((uint64_t)g.X * g.Y)) + 1;
+        const uint64_t c = g.cell(z, y, x);
+        const double r = (double)rho[c];
+        acc[0] += r;
+        for (int d = 0; d < 3; ++d) acc[1 + d] += r * (double)u[d * g.ncell + c];
+    }
+    for (int d = 0; d < 4; ++d) sh[d][threadIdx.x] = acc[d];
+    __syncthreads();
+    for (unsigned s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int d = 0; d < 4; ++d) sh[d][threadIdx.x] += sh[d][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 4; ++d) partial[4 * blockIdx.x + d] = sh[d][0];
+}
+
+__global__ void k_global_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a[4] = {0, 0, 0, 0};
+    for (int b = 0; b < nblocks; ++b)
+        for (int d = 0; d < 4; ++d) a[d] += partial[4 * b + d];
+    for (int d = 0; d < 4; ++d) out[d] = a[d];
+}
+
+int ensure_scratch(plb_state* s, size_t bytes) {
+    if (s->scratch_bytes >= bytes) return PLB_OK;
+    if (s->scratch) PLB_HIP_TRY(hipFree(s->scratch));
+    s->scratch = nullptr;
+    s->scratch_bytes = 0;
+    PLB_HIP_TRY(hipMalloc(&s->scratch, bytes));
+    s->scratch_bytes = bytes;
+    return PLB_OK;
+}
+
+dim3 interior_grid(const plb_state* s) {
+    return dim3((unsigned)((s->X + 63) / 64), (unsigned)s->Y, (unsigned)s->Z);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers ----
+int lit_initialize(plb_state* s, double rho0, const double u0[3], const plb_init_spec* init,
+                   const double* tg_dev) {
+    const LitDims g = dims_of(s);
+    const int kind = init ? init->kind : PLB_INIT_UNIFORM;
+    const uint64_t seed = init ? init->seed : 0;
+    const double amp = init ? init->amp : 0.0;
+    return with_qr(s, [&](auto qc, auto rr) {
+        constexpr int Q = decltype(qc)::value;
+        using R = decltype(rr);
+        hipLaunchKernelGGL((k_init<Q, R>), dim3(blocks_for(g.ncell, kThreads)), dim3(kThreads), 0,
+                           s->stream, g, (R*)s->buf[s->cur], (R*)s->buf[s->cur ^ 1], (R*)s->rho,
+                           (R*)s->u, rho0, u0[0], u0[1], u0[2], kind, seed, amp, tg_dev);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+}
+
+int lit_collide(plb_state* s, double tau, const double F[3]) {
+    const LitDims g = dims_of(s);
+    return with_qr(s, [&](auto qc, auto rr) {
+        constexpr int Q = decltype(qc)::value;
+        using R = decltype(rr);
+        R tr, om, omm;
+        if (std::is_same<R, float>::value) {  // CollisionSRT.cpp:19-20
+            tr = (R)tau;
+            om = (R)(1. / (double)tr);
+        } else {
+            tr = (R)tau;
+            om = (R)(1.0 / tau);
+        }
+        omm = (R)1 - om;
+        hipLaunchKernelGGL((k_collide<Q, R>), interior_grid(s), dim3(64), 0, s->stream, g,
+                           (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, tr, om, omm,
+                           (R)F[0], (R)F[1], (R)F[2]);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+}
+
+int lit_push(plb_state* s) {
+    const LitDims g = dims_of(s);
+    const int rc = with_qr(s, [&](auto qc, auto rr) {
+        constexpr int Q = decltype(qc)::value;
+        using R = decltype(rr);
+        hipLaunchKernelGGL((k_push<Q, R>), interior_grid(s), dim3(64), 0, s->stream, g,
+                           (const R*)s->buf[s->cur], (R*)s->buf[s->cur ^ 1]);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+    if (rc == PLB_OK) s->cur ^= 1;  // std::swap(simdata.ntmp, simdata.n)
+    return rc;
+}
+
+int lit_noslip(plb_state* s, const plb_boundary* bc) {
+    const LitDims g = dims_of(s);
+    for (int f = 0; f < 6; ++f) {
+        if (bc->type[f] != kNoslip) continue;
+        if (!bc->has_vel[f])
+            return fail(PLB_ERR_OUT_OF_RANGE, "no-slip face without a prescribed velocity");
+        const FaceBox b = face_box(f, s->X, s->Y, s->Z);
+        const int rc = with_qr(s, [&](auto qc, auto rr) {
+            constexpr int Q = decltype(qc)::value;
+            using R = decltype(rr);
+            const uint64_t total = b.count() * (Q - 1);
+            int e = ensure_scratch(s, total * sizeof(R));
+            if (e) return e;
+            const R cs2inv = (R)1 / (R)(1.0 / 3.0);
+            hipLaunchKernelGGL((k_noslip_gather<Q, R>), dim3(blocks_for(total, kThreads)),
+                               dim3(kThreads), 0, s->stream, g, b, (const R*)s->buf[s->cur],
+                               (const R*)s->rho, (R)bc->vel[f][0], (R)bc->vel[f][1],
+                               (R)bc->vel[f][2], cs2inv, (R*)s->scratch, total);
+            PLB_HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL((k_noslip_scatter<Q, R>), dim3(blocks_for(total, kThreads)),
+                               dim3(kThreads), 0, s->stream, g, b, (R*)s->buf[s->cur],
+                               (const R*)s->scratch, total);
+            PLB_HIP_TRY(hipGetLastError());
+            return PLB_OK;
+        });
+        if (rc) return rc;
+    }
+    return PLB_OK;
+}
+
+int lit_periodic(plb_state* s, const plb_boundary* bc) {
+    const LitDims g = dims_of(s);
+    for (int axis = 0; axis < 2; ++axis) {
+        if (!(bc->type[2 * axis] == kPeriodic && bc->type[2 * axis + 1] == kPeriodic)) continue;
+        const int rc = with_qr(s, [&](auto qc, auto rr) {
+            constexpr int Q = decltype(qc)::value;
+            using R = decltype(rr);
+            const uint64_t total = (uint64_t)s->Z * (axis == 0 ? s->Y : s->X) * (Q - 1);
+            hipLaunchKernelGGL((k_periodic<Q, R>), dim3(blocks_for(total, kThreads)), dim3(kThreads),
+                               0, s->stream, g, axis, (R*)s->buf[s->cur]);
+            PLB_HIP_TRY(hipGetLastError());
+            return PLB_OK;
+        });
+        if (rc) return rc;
+    }
+    return PLB_OK;
+}
+
+int lit_moments(plb_state* s) {
+    const LitDims g = dims_of(s);
+    return with_qr(s, [&](auto qc, auto rr) {
+        constexpr int Q = decltype(qc)::value;
+        using R = decltype(rr);
+        hipLaunchKernelGGL((k_moments<Q, R>), interior_grid(s), dim3(64), 0, s->stream, g,
+                           (const R*)s->buf[s->cur], (R*)s->rho, (R*)s->u);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+}
+
+int lit_reset_velocity(plb_state* s, const plb_boundary* bc) {
+    const LitDims g = dims_of(s);
+    for (int f = 0; f < 6; ++f) {
+        if (!bc->has_vel[f]) continue;
+        const FaceBox b = face_box(f, s->X, s->Y, s->Z);
+        const int rc = with_qr(s, [&](auto, auto rr) {
+            using R = decltype(rr);
+            hipLaunchKernelGGL((k_set_face<R>), dim3(blocks_for(b.count(), kThreads)), dim3(kThreads),
+                               0, s->stream, g, b, (R*)s->u, 3, (R)bc->vel[f][0], (R)bc->vel[f][1],
+                               (R)bc->vel[f][2]);
+            PLB_HIP_TRY(hipGetLastError());
+            return PLB_OK;
+        });
+        if (rc) return rc;
+    }
+    return PLB_OK;
+}
+
+int lit_reset_density(plb_state* s, const plb_boundary* bc) {
+    const LitDims g = dims_of(s);
+    for (int f = 0; f < 6; ++f) {
+        if (bc->type[f] != kNoslip) continue;
+        const FaceBox b = face_box(f, s->X, s->Y, s->Z);
+        const int rc = with_qr(s, [&](auto, auto rr) {
+            using R = decltype(rr);
+            hipLaunchKernelGGL((k_set_face<R>), dim3(blocks_for(b.count(), kThreads)), dim3(kThreads),
+                               0, s->stream, g, b, (R*)s->rho, 1, (R)bc->rho_s, (R)0, (R)0);
+            PLB_HIP_TRY(hipGetLastError());
+            return PLB_OK;
+        });
+        if (rc) return rc;
+    }
+    return PLB_OK;
+}
+
+int lit_transfer(plb_state* s, int field, int layout, int dtype, void* host, bool upload) {
+    const LitDims g = dims_of(s);
+    int vlen;
+    void* dev;
+    switch (field) {
+        case PLB_FIELD_RHO: vlen = 1; dev = s->rho; break;
+        case PLB_FIELD_U: vlen = 3; dev = s->u; break;
+        case PLB_FIELD_N: vlen = s->q; dev = s->buf[s->cur]; break;
+        case PLB_FIELD_NTMP: vlen = s->q; dev = s->buf[s->cur ^ 1]; break;
+        default: return fail(PLB_ERR_INVALID_ARGUMENT, "unknown field");
+    }
+    const int compact = layout == PLB_LAYOUT_COMPACT;
+    const uint64_t cells = compact ? (uint64_t)s->X * s->Y * s->Z : s->ncell;
+    const uint64_t total = cells * vlen;
+    const size_t hbytes = total * (dtype == PLB_DTYPE_F64 ? 8 : 4);
+    void* stage = nullptr;
+    PLB_HIP_TRY(hipMalloc(&stage, hbytes));
+    int rc = PLB_OK;
+    if (upload) {
+        hipError_t e = hipMemcpyAsync(stage, host, hbytes, hipMemcpyHostToDevice, s->stream);
+        if (e != hipSuccess) rc = fail(PLB_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+    if (rc == PLB_OK) {
+        rc = with_qr(s, [&](auto, auto rr) {
+            using R = decltype(rr);
+            if (dtype == PLB_DTYPE_F64)
+                hipLaunchKernelGGL((k_transfer<double, R>), dim3(blocks_for(total, kThreads)),
+                                   dim3(kThreads), 0, s->stream, g, (double*)stage, (R*)dev, vlen,
+                                   compact, upload ? 1 : 0, total);
+            else
+                hipLaunchKernelGGL((k_transfer<float, R>), dim3(blocks_for(total, kThreads)),
+                                   dim3(kThreads), 0, s->stream, g, (float*)stage, (R*)dev, vlen,
+                                   compact, upload ? 1 : 0, total);
+            PLB_HIP_TRY(hipGetLastError());
+            return PLB_OK;
+        });
+    }
+    if (rc == PLB_OK && !upload) {
+        hipError_t e = hipMemcpyAsync(host, stage, hbytes, hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) rc = fail(PLB_ERR_HIP, std::string("download: ") + hipGetErrorString(e));
+    }
+    hipError_t e = hipStreamSynchronize(s->stream);
+    (void)hipFree(stage);
+    if (rc == PLB_OK && e != hipSuccess) rc = fail(PLB_ERR_HIP, hipGetErrorString(e));
+    return rc;
+}
+
+int lit_global(plb_state* s, double out[4]) {
+    const LitDims g = dims_of(s);
+    const int nb = 512;
+    int rc = ensure_scratch(s, (size_t)(4 * nb + 4) * sizeof(double));
+    if (rc) return rc;
+    double* partial = (double*)s->scratch;
+    rc = with_qr(s, [&](auto, auto rr) {
+        using R = decltype(rr);
+        hipLaunchKernelGGL((k_global_partial<R>), dim3(nb), dim3(kThreads), 0, s->stream, g,
+                           (const R*)s->rho, (const R*)s->u, partial);
+        PLB_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_global_final, dim3(1), dim3(64), 0, s->stream, partial, nb, partial + 4 * nb);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+    if (rc) return rc;
+    PLB_HIP_TRY(hipMemcpyAsync(out, partial + 4 * nb, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PLB_HIP_TRY(hipStreamSynchronize(s->stream));
+    return PLB_OK;
+}
+
+}  // namespace plb

@@ -1,0 +1,300 @@
+"""GPU parity: the HIP library against the CPU oracle, through the C ABI.
+
+Literal mode (the reference's staged step) must match the oracle bit for bit
+in FP64 and in the reference's FP32 rounding, after every stage.  Fused mode
+must match the oracle's corrected step bit for bit for every storage variant
+(same expression order on both sides, -ffp-contract=off on both compilers).
+"""
+import numpy as np
+import pytest
+
+import plbpd_amd as P
+from oracle.pyoracle import Boundary, NOSLIP, PERIODIC, UNSET, UP, DOWN, NORTH, SOUTH, EAST, WEST
+
+pytestmark = pytest.mark.gpu
+
+LITERAL_CASES = [  # q, X, Y, Z
+    (9, 8, 1, 6),
+    (19, 5, 4, 3),
+    (19, 8, 6, 5),
+    (27, 5, 4, 3),
+    (27, 7, 5, 6),
+]
+
+
+def both_bc(types, vel, rho_s=1.0):
+    """The same boundary for the oracle and the library."""
+    return Boundary(type=list(types), vel=dict(vel), rho_s=rho_s), P.boundary(types, vel, rho_s)
+
+
+def couette_types(q):
+    if q == 9:
+        return [PERIODIC, PERIODIC, UNSET, UNSET, NOSLIP, NOSLIP]
+    return [PERIODIC, PERIODIC, PERIODIC, PERIODIC, NOSLIP, NOSLIP]
+
+
+def lit_compare(st, orc_st, where):
+    q = orc_st["q"]
+    for name, field in (("rho", P.FIELD_RHO), ("u", P.FIELD_U), ("n", P.FIELD_N), ("ntmp", P.FIELD_NTMP)):
+        dt = orc_st[name].dtype
+        got = st.download(field, P.LAYOUT_REFERENCE, dtype=dt)
+        exp = orc_st[name]
+        if not np.array_equal(got, exp):
+            bad = np.argwhere(got != exp)
+            raise AssertionError(f"{where}: field {name} (q={q}) differs at {len(bad)} entries, first "
+                                 f"{bad[:5].tolist()}: got {got[tuple(bad[0])]!r} expected {exp[tuple(bad[0])]!r}")
+
+
+@pytest.mark.parametrize("q,X,Y,Z", LITERAL_CASES)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_literal_stages_bitexact(orc, q, X, Y, Z, prec):
+    """Every stage of PLBPD::march_in_time, 3 steps, moving lid, perturbed start."""
+    print(f"\n[case] literal stages q={q} {X}x{Y}x{Z} {prec}", flush=True)
+    dt = np.float64 if prec == "f64" else np.float32
+    types = couette_types(q)
+    ob, pb = both_bc(types, {UP: (0.05, 0.01 if q != 9 else 0.0, 0.0), DOWN: (0.0, 0.0, 0.0)})
+    tau, F = 0.7, (1e-4, -2e-5 if q != 9 else 0.0, 3e-5)
+    o = orc.lit_state(q, X, Y, Z, dt)
+    orc.lit_init(o, 1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 1234, 1e-2), ob)
+    with P.State(q, X, Y, Z, prec, P.LITERAL) as st:
+        st.initialize(1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 1234, 1e-2), pb)
+        lit_compare(st, o, "initialize")
+        for step in range(3):
+            orc.lit_collide(o, tau, F)
+            st.collide(tau, F)
+            lit_compare(st, o, f"step {step} collide")
+            orc.lit_push(o)
+            st.stream("push")
+            lit_compare(st, o, f"step {step} push")
+            assert orc.lit_noslip(o, ob) == 0
+            st.apply_noslip(pb)
+            lit_compare(st, o, f"step {step} noslip")
+            orc.lit_periodic(o, ob)
+            st.apply_periodicity(pb)
+            lit_compare(st, o, f"step {step} periodic")
+            orc.lit_moments(o)
+            st.calc_moments()
+            lit_compare(st, o, f"step {step} moments")
+            orc.lit_reset(o, ob)
+            st.reset(pb)
+            lit_compare(st, o, f"step {step} reset")
+
+
+@pytest.mark.parametrize("q,X,Y,Z", LITERAL_CASES)
+def test_literal_run_bitexact(orc, q, X, Y, Z):
+    """plb_step (the whole loop body) over 40 steps, FP64."""
+    print(f"\n[case] literal run q={q} {X}x{Y}x{Z}", flush=True)
+    types = couette_types(q)
+    ob, pb = both_bc(types, {UP: (0.02, 0.0, 0.0), DOWN: (0.0, 0.0, 0.0)})
+    tau, F = 0.9, (1e-5, 0.0, 0.0)
+    o = orc.lit_state(q, X, Y, Z)
+    orc.lit_init(o, 1.0, (0, 0, 0), (P.INIT_PERTURBED, 7, 1e-3), ob)
+    orc.lit_run(o, tau, F, ob, 40)
+    with P.State(q, X, Y, Z, "f64", P.LITERAL) as st:
+        st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 7, 1e-3), pb)
+        st.configure(tau, F, pb)
+        st.step(40)
+        lit_compare(st, o, "after 40 steps")
+
+
+def test_eqlbdist_golden_on_gpu():
+    """TestEqlbDist.cpp:11-35: D2Q9 feq at rho=1, u=(0.5,0.5,-0.5) equals the
+    reference's nine float literals exactly, here produced by the GPU init."""
+    import json, os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+    exp = np.array(g["eqlbdist_d2q9"]["expected"], np.float32)
+    with P.State(9, 1, 1, 1, "f32", P.LITERAL) as st:
+        st.initialize(1.0, (0.5, 0.5, -0.5))
+        n = st.download(P.FIELD_N, P.LAYOUT_REFERENCE, dtype=np.float32)
+    assert np.array_equal(n[1, 1, 1], exp)
+    assert np.array_equal(n[0, 0, 0], exp)  # halo cells too (Domain.cpp:52-63)
+
+
+def test_literal_poiseuille_c1_parity(orc):
+    """D2Q9 256x64 channel, FP64, 10,000 steps: GPU literal == oracle literal."""
+    X, Y, Z = 256, 1, 64
+    print("\n[case] c1 D2Q9 256x1x64 10000 steps", flush=True)
+    ob, pb = both_bc([PERIODIC, PERIODIC, UNSET, UNSET, NOSLIP, NOSLIP],
+                     {UP: (0, 0, 0), DOWN: (0, 0, 0)})
+    tau, F, steps = 0.8, (1e-6, 0.0, 0.0), 10000
+    o = orc.lit_state(9, X, Y, Z)
+    orc.lit_init(o, 1.0, (0, 0, 0), None, ob)
+    orc.lit_run(o, tau, F, ob, steps)
+    with P.State(9, X, Y, Z, "f64", P.LITERAL) as st:
+        st.initialize(1.0, (0, 0, 0), None, pb)
+        st.configure(tau, F, pb)
+        st.step(steps)
+        rho = st.download(P.FIELD_RHO)
+        u = st.download(P.FIELD_U)
+    interior = (slice(1, -1), slice(1, -1), slice(1, -1))
+    r_rel = np.max(np.abs(rho[interior] - o["rho"][interior])) / np.max(np.abs(o["rho"][interior]))
+    u_rel = np.max(np.abs(u[interior] - o["u"][interior])) / np.max(np.abs(o["u"][interior]))
+    print(f"c1 max rel diff rho {r_rel:.3e} u {u_rel:.3e}")
+    assert r_rel <= 1e-12 and u_rel <= 1e-12
+    assert np.array_equal(rho, o["rho"]) and np.array_equal(u, o["u"])
+
+
+@pytest.mark.parametrize("N,steps", [(24, 200)])
+def test_literal_c2_reduced_parity(orc, N, steps):
+    """D3Q19 channel (periodic x and y, walls up/down), seeded 1e-4
+    perturbation, tau 0.6, F 1e-7: GPU literal == oracle literal."""
+    print(f"\n[case] c2 reduced {N}^3 {steps} steps", flush=True)
+    ob, pb = both_bc(couette_types(19), {UP: (0, 0, 0), DOWN: (0, 0, 0)})
+    tau, F = 0.6, (1e-7, 0.0, 0.0)
+    o = orc.lit_state(19, N, N, N)
+    orc.lit_init(o, 1.0, (0, 0, 0), (P.INIT_PERTURBED, 2024, 1e-4), ob)
+    orc.lit_run(o, tau, F, ob, steps)
+    with P.State(19, N, N, N, "f64", P.LITERAL) as st:
+        st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 2024, 1e-4), pb)
+        st.configure(tau, F, pb)
+        st.step(steps)
+        assert np.array_equal(st.download(P.FIELD_RHO), o["rho"])
+        assert np.array_equal(st.download(P.FIELD_U), o["u"])
+        assert np.array_equal(st.download(P.FIELD_N), o["n"])
+
+
+FUSED_CASES = [  # q, X, Y, Z, walls?, solids?
+    (9, 16, 1, 12, True, False),
+    (19, 16, 12, 10, True, False),
+    (19, 64, 8, 9, True, True),
+    (19, 12, 10, 8, False, False),
+    (27, 16, 10, 9, True, False),
+    (27, 10, 9, 8, True, True),
+]
+
+
+def fused_bc(q, walls, lid=(0.03, 0.0, 0.0)):
+    if walls:
+        t = couette_types(q)
+        v = {UP: lid, DOWN: (0.0, 0.0, 0.0)}
+    else:
+        t = [PERIODIC] * 6 if q != 9 else [PERIODIC, PERIODIC, UNSET, UNSET, PERIODIC, PERIODIC]
+        v = {}
+    return both_bc(t, v)
+
+
+@pytest.mark.parametrize("q,X,Y,Z,walls,solids", FUSED_CASES)
+@pytest.mark.parametrize("variant", ["f64", "f32", "f64a32", "f32s", "f16s"])
+def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant):
+    print(f"\n[case] fused {variant} q={q} {X}x{Y}x{Z} walls={walls} solids={solids}", flush=True)
+    ob, pb = fused_bc(q, walls)
+    tau, F = 0.62, (2e-5, 1e-6 if q != 9 else 0.0, -1e-6)
+    mask = None
+    if solids:
+        rng = np.random.default_rng(5)
+        mask = (rng.random((Z, Y, X)) < 0.1).astype(np.uint8)
+    init = (P.INIT_PERTURBED, 99, 2e-3)
+    o = orc.cor_state(variant, q, X, Y, Z)
+    orc.cor_init(o, 1.0, (0.01, 0, 0), init)
+    orc.cor_run(o, tau, F, ob, 25, mask)
+    with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+        st.set_boundary(pb, mask)
+        st.initialize(1.0, (0.01, 0, 0), init, pb)
+        st.configure(tau, F, pb)
+        st.step(25)
+        raw = st.download_raw()
+        rho, u = st.moments()
+    orho, ou = orc.cor_moments(o, ob, mask)
+    fl = orc.cor_flags(q, X, Y, Z, ob, mask)
+    fluid = fl == 0
+    if variant == "f16s":
+        assert np.array_equal(raw.view(np.uint16)[fluid], o["f"].view(np.uint16)[fluid])
+    else:
+        assert np.array_equal(raw[fluid], o["f"][fluid])
+    assert np.array_equal(rho, orho) and np.array_equal(u, ou)
+
+
+@pytest.mark.parametrize("variant", ["f64", "f32s"])
+def test_dropin_call_sequence_fused(orc, variant):
+    """The reference call sequence (collide, stream, noslip, periodicity,
+    moments, reset) on a fused state equals plb_step."""
+    q, X, Y, Z = 19, 16, 8, 10
+    ob, pb = fused_bc(q, True)
+    tau, F = 0.7, (1e-5, 0, 0)
+    with P.State(q, X, Y, Z, variant, P.FUSED) as a, P.State(q, X, Y, Z, variant, P.FUSED) as b:
+        for st in (a, b):
+            st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 3, 1e-3), pb)
+        b.configure(tau, F, pb)
+        for _ in range(6):
+            a.collide(tau, F)
+            a.stream("push")
+            a.apply_noslip(pb)
+            a.apply_periodicity(pb)
+            a.calc_moments()
+            a.reset(pb)
+        b.step(6)
+        assert np.array_equal(a.download_raw(), b.download_raw())
+
+
+def test_graph_replay_matches_eager():
+    """hipGraph replays (with eager work and syncs between) == plain launches."""
+    q, X, Y, Z = 19, 32, 16, 12
+    _, pb = fused_bc(q, True)
+    with P.State(q, X, Y, Z, "f32s", P.FUSED) as a, P.State(q, X, Y, Z, "f32s", P.FUSED) as b:
+        for st in (a, b):
+            st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 11, 1e-3), pb)
+            st.configure(0.8, (1e-5, 0, 0), pb)
+        for rep in range(4):
+            a.step_graph(10, 10)
+            b.step(10)
+            a.synchronize()
+            assert np.array_equal(a.download_raw(), b.download_raw()), f"replay {rep}"
+            a.global_moments()  # eager work between replays
+        a.step_graph(13, 4)  # remainder path
+        b.step(13)
+        assert np.array_equal(a.download_raw(), b.download_raw())
+
+
+@pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
+def test_global_moments_fp64(orc, variant):
+    q, X, Y, Z = 19, 20, 12, 10
+    ob, pb = fused_bc(q, True, lid=(0.0, 0.0, 0.0))
+    o = orc.cor_state(variant, q, X, Y, Z)
+    orc.cor_init(o, 1.0, (0, 0, 0), (P.INIT_PERTURBED, 5, 1e-3))
+    orc.cor_run(o, 0.7, (1e-5, 0, 0), ob, 10)
+    f = orc.cor_populations(o)
+    fl = orc.cor_flags(q, X, Y, Z, ob)
+    c = orc.tables(q)[0]
+    fluid = f[fl == 0]
+    exp = np.array([fluid.sum(), (fluid * c[:, 0]).sum(), (fluid * c[:, 1]).sum(), (fluid * c[:, 2]).sum()])
+    with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+        st.set_boundary(pb)
+        st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 5, 1e-3), pb)
+        st.configure(0.7, (1e-5, 0, 0), pb)
+        st.step(10)
+        got = st.global_moments()
+    assert abs(got[0] - exp[0]) <= 1e-12 * abs(exp[0])
+    assert np.allclose(got[1:], exp[1:], rtol=1e-9, atol=1e-13)
+
+
+def test_error_codes():
+    with pytest.raises(P.InvalidArgument):
+        P.State(19, 0, 4, 4)
+    with pytest.raises(P.LogicError):
+        P.State(15, 4, 4, 4)
+    with pytest.raises(P.InvalidArgument):
+        P.State(19, 4, 4, 4, "f16s", P.LITERAL)
+    with P.State(19, 4, 4, 4, "f64", P.LITERAL) as st:
+        with pytest.raises(P.LogicError):
+            st.stream("pull")
+        with pytest.raises(P.LogicError):
+            st.stream("sideways")
+        bad = P.boundary([PERIODIC, PERIODIC, PERIODIC, PERIODIC, NOSLIP, NOSLIP], {UP: (0, 0, 0)})
+        with pytest.raises(P.OutOfRange):
+            st.apply_noslip(bad)
+    with P.State(19, 4, 4, 4, "f32", P.FUSED) as st:
+        with pytest.raises(P.LogicError):  # periodic on one side only
+            st.set_boundary(P.boundary([PERIODIC, NOSLIP, PERIODIC, PERIODIC, PERIODIC, PERIODIC],
+                                       {WEST: (0, 0, 0)}))
+
+
+def test_timers_accumulate():
+    _, pb = fused_bc(19, True)
+    with P.State(19, 8, 8, 8, "f64", P.LITERAL) as st:
+        st.initialize(1.0, (0, 0, 0), None, pb)
+        st.configure(0.8, (1e-5, 0, 0), pb)
+        st.enable_timers(True)
+        st.step(5)
+        t = st.stage_times()
+    for k in ("collide", "stream", "noslip", "periodicity", "moments"):
+        assert t[k] > 0.0
