@@ -195,6 +195,24 @@ int plb_synchronize(plb_state* s);
 int plb_enable_timers(plb_state* s, int on);
 int plb_stage_times(plb_state* s, double* ms_out);
 
+/* ---- measurement (no reference counterpart; the reference times its loop
+ * with tbb::tick_count, PLBPD.cpp:22, 31) ---------------------------------- */
+/* Runs nsteps as plb_step (or plb_step_graph when steps_per_graph > 0)
+ * between two HIP events on the state's stream; ms = elapsed time. */
+int plb_time_steps(plb_state* s, int nsteps, int steps_per_graph, double* ms);
+/* Copy-kernel calibration: copies `bytes` device-to-device `iters` times;
+ * gbps counts bytes read plus bytes written. */
+int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms_per_copy);
+/* Fused-kernel tuning knobs. */
+#define PLB_TUNE_BLOCK_X 0  /* cells per block row: 64, 128, 256 (0 = from x) */
+#define PLB_TUNE_NT_LOAD 1  /* non-temporal population loads (0/1)            */
+#define PLB_TUNE_NT_STORE 2 /* non-temporal population stores (0/1)           */
+#define PLB_TUNE_SCHEME 3   /* streaming scheme of the fused step:            */
+#define PLB_SCHEME_AUTO 0   /*   pull unless a wall moves                     */
+#define PLB_SCHEME_PUSH 1   /*   collide, then scatter to neighbours          */
+#define PLB_SCHEME_PULL 2   /*   gather from neighbours, then collide         */
+int plb_set_tuning(plb_state* s, int key, int value);
+
 /* ---- batched domains (no reference counterpart: one domain per PLBPD,
  * PLBPD.hpp:14-17) --------------------------------------------------------- */
 typedef struct {

@@ -29,6 +29,8 @@ FIELD_RHO, FIELD_U, FIELD_N, FIELD_NTMP = range(4)
 LAYOUT_REFERENCE, LAYOUT_COMPACT = 0, 1
 DTYPE_F64, DTYPE_F32 = 0, 1
 INIT_UNIFORM, INIT_PERTURBED, INIT_TAYLOR_GREEN = 0, 1, 2
+TUNE_BLOCK_X, TUNE_NT_LOAD, TUNE_NT_STORE, TUNE_SCHEME = 0, 1, 2, 3
+SCHEME_AUTO, SCHEME_PUSH, SCHEME_PULL = 0, 1, 2
 STAGES = ["collide", "stream", "noslip", "periodicity", "moments", "velocity_reset",
           "density_reset", "fused"]
 
@@ -146,6 +148,9 @@ def lib():
     L.plb_synchronize.argtypes = [vp]
     L.plb_enable_timers.argtypes = [vp, i]
     L.plb_stage_times.argtypes = [vp, vp]
+    L.plb_time_steps.argtypes = [vp, i, i, C.POINTER(C.c_double)]
+    L.plb_bench_copy.argtypes = [i, sz, i, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.plb_set_tuning.argtypes = [vp, i, i]
     if hasattr(L, "plb_batch_create"):
         L.plb_batch_create.argtypes = [i, i, i, C.POINTER(DomainDesc), C.POINTER(BoundaryC), i, C.POINTER(vp)]
         L.plb_batch_destroy.argtypes = [vp]
@@ -184,6 +189,13 @@ def device_info(device=0):
     mem = C.c_size_t(0)
     check(lib().plb_device_info(device, name, 256, C.byref(ncu), C.byref(mem)))
     return {"name": name.value.decode(), "num_cu": ncu.value, "hbm_bytes": mem.value}
+
+
+def bench_copy(nbytes=1 << 31, iters=20, device=0):
+    """Device copy bandwidth (GB/s, read + write) and ms per copy."""
+    g, ms = C.c_double(), C.c_double()
+    check(lib().plb_bench_copy(device, int(nbytes), int(iters), C.byref(g), C.byref(ms)))
+    return g.value, ms.value
 
 
 def lattice_tables(q: int):
@@ -296,6 +308,15 @@ class State:
 
     def step_graph(self, n, steps_per_graph=20):
         check(lib().plb_step_graph(self.h, int(n), int(steps_per_graph)))
+
+    def time_steps(self, n, steps_per_graph=0):
+        """Milliseconds for n steps, timed with HIP events on the state's stream."""
+        ms = C.c_double()
+        check(lib().plb_time_steps(self.h, int(n), int(steps_per_graph), C.byref(ms)))
+        return ms.value
+
+    def set_tuning(self, key, value):
+        check(lib().plb_set_tuning(self.h, int(key), int(value)))
 
     def global_moments(self):
         out = np.zeros(4, np.float64)

@@ -205,6 +205,11 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->graph_steps = 0;
     s->graph_parity = 0;
     s->timers_on = false;
+    s->tune_tx = 0;
+    s->tune_nt_load = 0;
+    s->tune_nt_store = 0;
+    s->tune_scheme = PLB_SCHEME_AUTO;
+    s->post_collision = false;
     for (double& v : s->stage_ms) v = 0.0;
 
     const uint64_t X = d->x, Y = d->y, Z = d->z;
@@ -484,6 +489,19 @@ int plb_step_graph(plb_state* s, int nsteps, int steps_per_graph) {
     if (steps_per_graph <= 0 || (steps_per_graph & 1))
         return fail(PLB_ERR_INVALID_ARGUMENT, "steps_per_graph must be positive and even");
     if (nsteps < 0) return fail(PLB_ERR_INVALID_ARGUMENT, "negative step count");
+    // The captured steps keep one storage form: post-collision for the pull
+    // step (from the pre-collision form its first step is the collision
+    // alone, run here), pre-collision for the push step.
+    if (fused_uses_pull(s)) {
+        if (!s->post_collision && nsteps > 0) {
+            rc = timed(s, PLB_STAGE_FUSED, [&] { return fused_launch(s, true); });
+            if (rc) return rc;
+            --nsteps;
+        }
+    } else {
+        rc = fused_to_pre(s);
+        if (rc) return rc;
+    }
     const int reps = nsteps / steps_per_graph;
     if (reps > 0 && (!s->graph || s->graph_steps != steps_per_graph || s->graph_parity != s->cur)) {
         drop_graph(s);

@@ -25,6 +25,7 @@ struct FusedArgs {
     uint64_t pstride;
     uint32_t tx_shift, tx_mask;  // TX = 1 << tx_shift
     uint32_t ntx, nty, ntiles;
+    int32_t nt_load, nt_store;  // non-temporal loads / stores
     T tau, om, omm, F0, F1, F2;
     FusedBC<T> bc;
 };
@@ -43,110 +44,236 @@ __device__ inline uint32_t xcd_tile(uint32_t i, uint32_t n) {
     return j * n8 + (j < r ? j : r) + (i >> 3);
 }
 
-template <int Q, int V, int NB, bool COLLIDE>
-__global__ __launch_bounds__(kBlock) void k_fused(const typename Prec<V>::S* __restrict__ A,
-                                                  typename Prec<V>::S* __restrict__ B,
-                                                  const uint8_t* __restrict__ flags,
-                                                  const uint32_t* __restrict__ nbmask,
-                                                  FusedArgs<typename Prec<V>::T> a) {
-    using P = Prec<V>;
-    using T = typename P::T;
+struct Cell {
+    uint32_t x, y, z;
+    uint64_t c;
+    bool valid;
+};
+
+// block -> tile -> (x, y, z) of this thread's cell
+template <class T>
+__device__ inline Cell cell_of(const FusedArgs<T>& a) {
     const uint32_t t = xcd_tile(blockIdx.x, a.ntiles);
     const uint32_t tx = t % a.ntx;
     const uint32_t rest = t / a.ntx;
     const uint32_t ty = rest % a.nty;
-    const uint32_t z = rest / a.nty;
-    const uint32_t x = (tx << a.tx_shift) + (threadIdx.x & a.tx_mask);
-    const uint32_t y = ty * (kBlock >> a.tx_shift) + (threadIdx.x >> a.tx_shift);
-    if (x >= a.X || y >= a.Y) return;
-    const uint64_t XY = (uint64_t)a.X * a.Y;
-    const uint64_t c = (uint64_t)z * XY + (uint64_t)y * a.X + x;
+    Cell p;
+    p.z = rest / a.nty;
+    p.x = (tx << a.tx_shift) + (threadIdx.x & a.tx_mask);
+    p.y = ty * (kBlock >> a.tx_shift) + (threadIdx.x >> a.tx_shift);
+    p.valid = p.x < a.X && p.y < a.Y;
+    p.c = (uint64_t)p.z * a.X * a.Y + (uint64_t)p.y * a.X + p.x;
+    return p;
+}
 
+// Moments (CalcMoments.hpp:27-35 order) and, when `collide`, the BGK update
+// with the velocity-shift force (CollisionSRT.cpp:72-91, 170-175) in place.
+// Returns rho.  Shifted storage sums deviations and adds 1 afterwards.
+template <int Q, class P>
+__device__ inline typename P::T bgk(typename P::T (&f)[Q], const FusedArgs<typename P::T>& a, bool collide) {
+    using T = typename P::T;
+    T r = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        r += f[k];
+        j0 += f[k] * (T)Lat<Q>::cx(k);
+        j1 += f[k] * (T)Lat<Q>::cy(k);
+        j2 += f[k] * (T)Lat<Q>::cz(k);
+    }
+    const T drho = r;
+    if (P::shifted) r = (T)1 + r;
+    if (!collide) return r;
+    const T ri = (T)1 / r;
+    const T tri = a.tau / r;
+    const T u0 = j0 * ri + a.F0 * tri;
+    const T u1 = j1 * ri + a.F1 * tri;
+    const T u2 = j2 * ri + a.F2 * tri;
+    const T usq = u0 * u0 + u1 * u1 + u2 * u2;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const T cu = (k == 0) ? (T)0
+                              : (T)Lat<Q>::cx(k) * u0 + (T)Lat<Q>::cy(k) * u1 + (T)Lat<Q>::cz(k) * u2;
+        const T w = (T)Lat<Q>::w(k);
+        if (P::shifted) {
+            const T geq = w * (drho + r * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
+            f[k] = a.omm * f[k] + a.om * geq;
+        } else {
+            const T neq = (w * r) * ((((T)1 + (T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq);
+            f[k] = a.omm * f[k] + a.om * neq;
+        }
+    }
+    return r;
+}
+
+template <class T>
+struct Nbr {
+    uint32_t xo[3], yo[3], zo[3];
+    bool wx[3], wy[3], wz[3];
+};
+
+// Neighbour coordinates with periodic wrap (a walled axis never wraps for a
+// fluid cell: the wall layer sits between it and the domain edge), and which
+// neighbour layers are wall layers.
+template <class T>
+__device__ inline Nbr<T> neighbours(const Cell& p, const FusedArgs<T>& a) {
+    Nbr<T> n;
+    n.xo[0] = p.x == 0 ? a.X - 1 : p.x - 1;
+    n.xo[1] = p.x;
+    n.xo[2] = p.x + 1 == a.X ? 0 : p.x + 1;
+    n.yo[0] = p.y == 0 ? a.Y - 1 : p.y - 1;
+    n.yo[1] = p.y;
+    n.yo[2] = p.y + 1 == a.Y ? 0 : p.y + 1;
+    n.zo[0] = p.z == 0 ? a.Z - 1 : p.z - 1;
+    n.zo[1] = p.z;
+    n.zo[2] = p.z + 1 == a.Z ? 0 : p.z + 1;
+    n.wx[0] = a.bc.wall_lo[0] && n.xo[0] == 0;
+    n.wx[1] = false;
+    n.wx[2] = a.bc.wall_hi[0] && n.xo[2] == a.X - 1;
+    n.wy[0] = a.bc.wall_lo[1] && n.yo[0] == 0;
+    n.wy[1] = false;
+    n.wy[2] = a.bc.wall_hi[1] && n.yo[2] == a.Y - 1;
+    n.wz[0] = a.bc.wall_lo[2] && n.zo[0] == 0;
+    n.wz[1] = false;
+    n.wz[2] = a.bc.wall_hi[2] && n.zo[2] == a.Z - 1;
+    return n;
+}
+
+// Push step: each fluid cell loads its own populations (aligned), collides,
+// and stores each one at the neighbour it moves to, or reflected into its own
+// reverse slot when that neighbour is solid (halfway bounce-back with the
+// moving-wall term of Boundary.cpp:213-217).
+template <int Q, int V, int NB, bool COLLIDE>
+__global__ __launch_bounds__(kBlock) void k_push(const typename Prec<V>::S* __restrict__ A,
+                                                 typename Prec<V>::S* __restrict__ B,
+                                                 const uint8_t* __restrict__ flags,
+                                                 const uint32_t* __restrict__ nbmask,
+                                                 FusedArgs<typename Prec<V>::T> a) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    const Cell p = cell_of(a);
+    if (!p.valid) return;
+    const uint64_t c = p.c;
     uint32_t m = 0;
+    bool self_solid;
     if (NB) {
         m = nbmask[c];
-        if (m & 1u) return;
+        self_solid = m & 1u;
     } else {
-        if (flags[c]) return;
+        self_solid = flags[c] != 0;
     }
+    // A wavefront made only of solid cells (a wall row) has nothing to do;
+    // otherwise solid lanes write zeros into their own slots.
+    if (__all(self_solid)) return;
 
     T f[Q];
+    if (self_solid) {
 #pragma unroll
-    for (int k = 0; k < Q; ++k) f[k] = P::load(A[(uint64_t)k * a.pstride + c]);
-
+        for (int k = 0; k < Q; ++k) f[k] = (T)0;
+    } else if (a.nt_load) {
+#pragma unroll
+        for (int k = 0; k < Q; ++k) f[k] = P::load(__builtin_nontemporal_load(A + (uint64_t)k * a.pstride + c));
+    } else {
+#pragma unroll
+        for (int k = 0; k < Q; ++k) f[k] = P::load(A[(uint64_t)k * a.pstride + c]);
+    }
     T rho = (T)0;
-    if (COLLIDE || a.bc.moving) {
-        T r = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            r += f[k];
-            j0 += f[k] * (T)Lat<Q>::cx(k);
-            j1 += f[k] * (T)Lat<Q>::cy(k);
-            j2 += f[k] * (T)Lat<Q>::cz(k);
-        }
-        const T drho = r;
-        if (P::shifted) r = (T)1 + r;
-        rho = r;
-        if (COLLIDE) {
-            const T ri = (T)1 / r;
-            const T tri = a.tau / r;
-            const T u0 = j0 * ri + a.F0 * tri;
-            const T u1 = j1 * ri + a.F1 * tri;
-            const T u2 = j2 * ri + a.F2 * tri;
-            const T usq = u0 * u0 + u1 * u1 + u2 * u2;
-#pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                const T cu = (k == 0) ? (T)0
-                                      : (T)Lat<Q>::cx(k) * u0 + (T)Lat<Q>::cy(k) * u1 +
-                                            (T)Lat<Q>::cz(k) * u2;
-                const T w = (T)Lat<Q>::w(k);
-                if (P::shifted) {
-                    const T geq = w * (drho + r * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
-                    f[k] = a.omm * f[k] + a.om * geq;
-                } else {
-                    const T neq = (w * r) * ((((T)1 + (T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq);
-                    f[k] = a.omm * f[k] + a.om * neq;
-                }
-            }
-        }
-    }
+    if (COLLIDE || a.bc.moving) rho = bgk<Q, P>(f, a, COLLIDE);
 
-    // neighbour coordinates with periodic wrap (walled axes never wrap for a
-    // fluid cell: the wall layer sits between it and the domain edge)
-    const uint32_t xo[3] = {x == 0 ? a.X - 1 : x - 1, x, x + 1 == a.X ? 0 : x + 1};
-    const uint32_t yo[3] = {y == 0 ? a.Y - 1 : y - 1, y, y + 1 == a.Y ? 0 : y + 1};
-    const uint32_t zo[3] = {z == 0 ? a.Z - 1 : z - 1, z, z + 1 == a.Z ? 0 : z + 1};
-    bool wx[3] = {false, false, false}, wy[3] = {false, false, false}, wz[3] = {false, false, false};
-    if (!NB) {
-        wx[0] = a.bc.wall_lo[0] && xo[0] == 0;
-        wx[2] = a.bc.wall_hi[0] && xo[2] == a.X - 1;
-        wy[0] = a.bc.wall_lo[1] && yo[0] == 0;
-        wy[2] = a.bc.wall_hi[1] && yo[2] == a.Y - 1;
-        wz[0] = a.bc.wall_lo[2] && zo[0] == 0;
-        wz[2] = a.bc.wall_hi[2] && zo[2] == a.Z - 1;
-    }
+    const Nbr<T> nb = neighbours(p, a);
+    const uint64_t XY = (uint64_t)a.X * a.Y;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
-        const uint64_t d = (uint64_t)zo[cz + 1] * XY + (uint64_t)yo[cy + 1] * a.X + xo[cx + 1];
-        bool solid;
-        if (NB)
-            solid = (m >> k) & 1u;
-        else
-            solid = wx[cx + 1] || wy[cy + 1] || wz[cz + 1];
-        if (solid) {
-            T v = f[k];
-            if (a.bc.moving) {
-                const int fl = flags[d];
-                const T cub = (T)cx * a.bc.vel[fl][0] + (T)cy * a.bc.vel[fl][1] + (T)cz * a.bc.vel[fl][2];
-                const T term = (T)2 * (T)Lat<Q>::w(k) * rho * cub * (T)3;
-                v = v - term;
-            }
-            B[(uint64_t)rev_k<Q>(k) * a.pstride + c] = P::store(v);
-        } else {
-            B[(uint64_t)k * a.pstride + d] = P::store(f[k]);
+        const uint64_t d = (uint64_t)nb.zo[cz + 1] * XY + (uint64_t)nb.yo[cy + 1] * a.X + nb.xo[cx + 1];
+        const bool solid = NB ? ((m >> k) & 1u) != 0 : (nb.wx[cx + 1] || nb.wy[cy + 1] || nb.wz[cz + 1]);
+        T v = f[k];
+        if (a.bc.moving && solid && !self_solid) {
+            const int fl = flags[d];
+            const T cub = (T)cx * a.bc.vel[fl][0] + (T)cy * a.bc.vel[fl][1] + (T)cz * a.bc.vel[fl][2];
+            const T term = (T)2 * (T)Lat<Q>::w(k) * rho * cub * (T)3;
+            v = v - term;
         }
+        // one store per direction, whatever the lane's case
+        uint64_t dst = solid ? (uint64_t)rev_k<Q>(k) * a.pstride + c : (uint64_t)k * a.pstride + d;
+        if (self_solid) {
+            dst = (uint64_t)k * a.pstride + c;
+            v = (T)0;
+        }
+        if (a.nt_store)
+            __builtin_nontemporal_store(P::store(v), B + dst);
+        else
+            B[dst] = P::store(v);
     }
+}
+
+// Pull step on post-collision storage (stationary walls): each fluid cell
+// gathers population k from x - c_k, or from its own reverse slot when that
+// cell is solid (the same halfway bounce-back as the push step), then
+// collides and stores every population in its own slot.  A wavefront writes
+// only its own row, so every cache line of B is written whole, by one
+// instruction, even around interior solids.  COLLIDE = false gathers only
+// (post-collision -> pre-collision populations).
+template <int Q, int V, int NB, bool COLLIDE>
+__global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __restrict__ A,
+                                                 typename Prec<V>::S* __restrict__ B,
+                                                 const uint8_t* __restrict__ flags,
+                                                 const uint32_t* __restrict__ nbmask,
+                                                 FusedArgs<typename Prec<V>::T> a) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    const Cell p = cell_of(a);
+    if (!p.valid) return;
+    const uint64_t c = p.c;
+    uint32_t m = 0;
+    bool self_solid;
+    if (NB) {
+        m = nbmask[c];
+        self_solid = m & 1u;
+    } else {
+        self_solid = flags[c] != 0;
+    }
+    if (__all(self_solid)) return;
+
+    const Nbr<T> nb = neighbours(p, a);
+    const uint64_t XY = (uint64_t)a.X * a.Y;
+    T f[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
+        // source x - c_k; it is solid when neighbour rev(k) is
+        const uint64_t s = (uint64_t)nb.zo[1 - cz] * XY + (uint64_t)nb.yo[1 - cy] * a.X + nb.xo[1 - cx];
+        const bool solid = NB ? ((m >> rev_k<Q>(k)) & 1u) != 0 : (nb.wx[1 - cx] || nb.wy[1 - cy] || nb.wz[1 - cz]);
+        const uint64_t src = solid ? (uint64_t)rev_k<Q>(k) * a.pstride + c : (uint64_t)k * a.pstride + s;
+        T v = (T)0;
+        if (!self_solid) v = a.nt_load ? P::load(__builtin_nontemporal_load(A + src)) : P::load(A[src]);
+        f[k] = v;
+    }
+    if (COLLIDE) bgk<Q, P>(f, a, true);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const T v = self_solid ? (T)0 : f[k];
+        if (a.nt_store)
+            __builtin_nontemporal_store(P::store(v), B + (uint64_t)k * a.pstride + c);
+        else
+            B[(uint64_t)k * a.pstride + c] = P::store(v);
+    }
+}
+
+// Collision only, in place (pre-collision -> post-collision storage).
+template <int Q, int V>
+__global__ __launch_bounds__(kBlock) void k_collide_only(typename Prec<V>::S* __restrict__ A,
+                                                         const uint8_t* __restrict__ flags,
+                                                         FusedArgs<typename Prec<V>::T> a) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    const Cell p = cell_of(a);
+    if (!p.valid || flags[p.c]) return;
+    T f[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) f[k] = P::load(A[(uint64_t)k * a.pstride + p.c]);
+    bgk<Q, P>(f, a, true);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) A[(uint64_t)k * a.pstride + p.c] = P::store(f[k]);
 }
 
 // Initial state: populations from feq(rho0, u(cell)) evaluated in double
@@ -455,6 +582,7 @@ int fused_initialize(plb_state* s, double rho0, const double u0[3], const plb_in
         constexpr int Q = decltype(qc)::value;
         constexpr int V = decltype(vc)::value;
         using S = typename Prec<V>::S;
+        s->post_collision = false;
         hipLaunchKernelGGL((k_finit<Q, V>), dim3(blocks_for(s->ncell, kBlock)), dim3(kBlock), 0,
                            s->stream, (S*)s->buf[s->cur], (uint32_t)s->X, (uint32_t)s->Y,
                            (uint32_t)s->Z, (uint64_t)s->pstride, rho0, u0[0], u0[1], u0[2], kind, seed,
@@ -464,10 +592,17 @@ int fused_initialize(plb_state* s, double rho0, const double u0[3], const plb_in
     });
 }
 
-int fused_launch(plb_state* s, bool collide) {
+namespace {
+
+enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly };
+
+// Launch geometry and arguments shared by every fused-layout kernel.
+int launch_kind(plb_state* s, KernelKind kind) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
     unsigned tx = 64;
-    if (s->X % 256 == 0)
+    if (s->tune_tx > 0)
+        tx = (unsigned)s->tune_tx;
+    else if (s->X % 256 == 0)
         tx = 256;
     else if (s->X % 128 == 0)
         tx = 128;
@@ -492,8 +627,10 @@ int fused_launch(plb_state* s, bool collide) {
         a.ntx = ntx;
         a.nty = nty;
         a.ntiles = (uint32_t)ntiles;
+        a.nt_load = s->tune_nt_load;
+        a.nt_store = s->tune_nt_store;
         a.tau = (T)s->tau;
-        if (std::is_same<T, float>::value)
+        if (std::is_same<T, float>::value)  // CollisionSRT.cpp:19-20
             a.om = (T)(1. / (double)a.tau);
         else
             a.om = (T)(1.0 / s->tau);
@@ -505,29 +642,94 @@ int fused_launch(plb_state* s, bool collide) {
         const S* A = (const S*)s->buf[s->cur];
         S* B = (S*)s->buf[s->cur ^ 1];
         const dim3 grid((unsigned)ntiles), block(kBlock);
-        if (s->has_solids) {
-            if (collide)
-                hipLaunchKernelGGL((k_fused<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-            else
-                hipLaunchKernelGGL((k_fused<Q, V, 1, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-        } else {
-            if (collide)
-                hipLaunchKernelGGL((k_fused<Q, V, 0, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-            else
-                hipLaunchKernelGGL((k_fused<Q, V, 0, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+        const bool nb = s->has_solids;
+        switch (kind) {
+            case kPush:
+                if (nb) hipLaunchKernelGGL((k_push<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                else hipLaunchKernelGGL((k_push<Q, V, 0, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                break;
+            case kStreamOnly:
+                if (nb) hipLaunchKernelGGL((k_push<Q, V, 1, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                else hipLaunchKernelGGL((k_push<Q, V, 0, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                break;
+            case kPull:
+                if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                else hipLaunchKernelGGL((k_pull<Q, V, 0, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                break;
+            case kGather:
+                if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                else hipLaunchKernelGGL((k_pull<Q, V, 0, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                break;
+            case kCollideOnly:
+                hipLaunchKernelGGL((k_collide_only<Q, V>), grid, block, 0, s->stream, (S*)s->buf[s->cur],
+                                   (const uint8_t*)s->flags, a);
+                break;
         }
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
-    if (rc == PLB_OK) {
-        s->cur ^= 1;
-        s->moments_valid = false;
+    if (rc == PLB_OK && kind != kCollideOnly) s->cur ^= 1;
+    return rc;
+}
+
+bool moving_walls(const plb_state* s) {
+    for (int f = 0; f < 6; ++f)
+        if (s->bc.type[f] == kNoslip)
+            for (int d = 0; d < 3; ++d)
+                if (s->bc.vel[f][d] != 0.0) return true;
+    return false;
+}
+
+}  // namespace
+
+bool fused_uses_pull(const plb_state* s) {
+    if (s->tune_scheme == PLB_SCHEME_PUSH) return false;
+    if (moving_walls(s)) return false;  // the pull step has no moving-wall term
+    if (s->tune_scheme == PLB_SCHEME_PULL) return true;
+    // Automatic choice, from measurement on MI355X (D3Q19 256^3): the pull
+    // step wins for 4- and 8-byte storage (whole-row stores, no partial
+    // lines around solids); with 2-byte storage its x-shifted loads span two
+    // cache lines per wavefront and the push step is faster.
+    return s->sbytes >= 4;
+}
+
+// The populations in buf[cur] are stored either before collision (what the
+// reference's n holds after a whole step) or, between pull steps, after the
+// collision of the step in progress, whose streaming is still pending.  A
+// pull-mode step from the pre-collision form is the collision alone; every
+// later step gathers (finishing the pending streaming), then collides.
+// Reading the state performs the pending gather first.
+int fused_to_pre(plb_state* s) {
+    if (!s->post_collision) return PLB_OK;
+    const int rc = launch_kind(s, kGather);
+    if (rc == PLB_OK) s->post_collision = false;
+    return rc;
+}
+
+int fused_launch(plb_state* s, bool collide) {
+    int rc;
+    if (!collide) {
+        rc = fused_to_pre(s);
+        if (rc == PLB_OK) rc = launch_kind(s, kStreamOnly);
+    } else if (fused_uses_pull(s)) {
+        if (!s->post_collision) {
+            rc = launch_kind(s, kCollideOnly);
+            if (rc == PLB_OK) s->post_collision = true;
+        } else {
+            rc = launch_kind(s, kPull);
+        }
+    } else {
+        rc = fused_to_pre(s);
+        if (rc == PLB_OK) rc = launch_kind(s, kPush);
     }
+    s->moments_valid = false;
     return rc;
 }
 
 int fused_moments(plb_state* s) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "moments before the boundary was set");
+    int prc = fused_to_pre(s);
+    if (prc) return prc;
     FusedBC<double> bc;
     make_fused_bc<double>(s->bc, s->q, &bc);
     const int rc = with_qv(s->q, s->variant, [&](auto qc, auto vc) {
@@ -559,6 +761,10 @@ int fused_transfer(plb_state* s, int field, int layout, int dtype, void* host, b
     const size_t hbytes = hcells * vlen * esz;
     if (field != PLB_FIELD_N && !s->moments_valid) {
         const int rc = fused_moments(s);
+        if (rc) return rc;
+    }
+    if (field == PLB_FIELD_N && !upload) {
+        const int rc = fused_to_pre(s);
         if (rc) return rc;
     }
     void* stage = nullptr;
@@ -607,11 +813,18 @@ int fused_transfer(plb_state* s, int field, int layout, int dtype, void* host, b
     hipError_t e = hipStreamSynchronize(s->stream);
     (void)hipFree(stage);
     if (rc == PLB_OK && e != hipSuccess) rc = fail(PLB_ERR_HIP, hipGetErrorString(e));
-    if (rc == PLB_OK && upload) s->moments_valid = false;
+    if (rc == PLB_OK && upload) {
+        s->moments_valid = false;
+        s->post_collision = false;  // uploaded populations are pre-collision
+    }
     return rc;
 }
 
 int fused_download_raw(plb_state* s, void* host, size_t bytes) {
+    if (s->bc_set) {
+        const int prc = fused_to_pre(s);
+        if (prc) return prc;
+    }
     const size_t need = s->ncell * s->q * s->sbytes;
     if (bytes < need) return fail(PLB_ERR_INVALID_ARGUMENT, "raw buffer too small");
     void* stage = nullptr;
@@ -640,6 +853,8 @@ int fused_download_raw(plb_state* s, void* host, size_t bytes) {
 
 int fused_global(plb_state* s, double out[4]) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "global moments before the boundary was set");
+    int prc = fused_to_pre(s);
+    if (prc) return prc;
     const int nb = 1024;
     int rc = ensure_scratch(s, (size_t)(4 * nb + 4) * sizeof(double));
     if (rc) return rc;

@@ -77,6 +77,11 @@ struct plb_state {
     hipGraphExec_t graph;
     int graph_steps;
     int graph_parity;
+    // tuning knobs of the fused kernel (plb_set_tuning)
+    int tune_tx;  // block width in x (64, 128, 256); 0 = pick from X
+    int tune_nt_load, tune_nt_store;
+    int tune_scheme;      // PLB_SCHEME_*
+    bool post_collision;  // fused: buf[cur] holds post-collision populations (pull step)
     // timers
     bool timers_on;
     std::vector<plb::TimerPair> pending;
@@ -108,6 +113,8 @@ int fused_initialize(plb_state* s, double rho0, const double u0[3], const plb_in
                      const double* tg_dev);
 int fused_build_flags(plb_state* s, const plb_boundary* bc, const uint8_t* mask);
 int fused_launch(plb_state* s, bool collide);
+bool fused_uses_pull(const plb_state* s);
+int fused_to_pre(plb_state* s);
 int fused_moments(plb_state* s);
 int fused_transfer(plb_state* s, int field, int layout, int dtype, void* host, bool upload);
 int fused_download_raw(plb_state* s, void* host, size_t bytes);

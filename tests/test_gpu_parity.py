@@ -161,6 +161,7 @@ FUSED_CASES = [  # q, X, Y, Z, walls?, solids?
     (27, 16, 10, 9, True, False),
     (27, 10, 9, 8, True, True),
 ]
+SCHEMES = {"push": 1, "pull": 2}
 
 
 def fused_bc(q, walls, lid=(0.03, 0.0, 0.0)):
@@ -175,9 +176,11 @@ def fused_bc(q, walls, lid=(0.03, 0.0, 0.0)):
 
 @pytest.mark.parametrize("q,X,Y,Z,walls,solids", FUSED_CASES)
 @pytest.mark.parametrize("variant", ["f64", "f32", "f64a32", "f32s", "f16s"])
-def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant):
-    print(f"\n[case] fused {variant} q={q} {X}x{Y}x{Z} walls={walls} solids={solids}", flush=True)
-    ob, pb = fused_bc(q, walls)
+@pytest.mark.parametrize("scheme", ["push", "pull"])
+def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
+    print(f"\n[case] fused {scheme} {variant} q={q} {X}x{Y}x{Z} walls={walls} solids={solids}", flush=True)
+    lid = (0.03, 0.0, 0.0) if scheme == "push" else (0.0, 0.0, 0.0)  # pull: stationary walls
+    ob, pb = fused_bc(q, walls, lid)
     tau, F = 0.62, (2e-5, 1e-6 if q != 9 else 0.0, -1e-6)
     mask = None
     if solids:
@@ -188,6 +191,7 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant):
     orc.cor_init(o, 1.0, (0.01, 0, 0), init)
     orc.cor_run(o, tau, F, ob, 25, mask)
     with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+        st.set_tuning(P.capi.TUNE_SCHEME, SCHEMES[scheme])
         st.set_boundary(pb, mask)
         st.initialize(1.0, (0.01, 0, 0), init, pb)
         st.configure(tau, F, pb)
@@ -202,6 +206,33 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant):
     else:
         assert np.array_equal(raw[fluid], o["f"][fluid])
     assert np.array_equal(rho, orho) and np.array_equal(u, ou)
+
+
+@pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
+def test_pull_push_same_trajectory_with_midrun_reads(variant):
+    """Pull and push steps give identical populations, and reading the state
+    mid-run (which converts post- to pre-collision storage) changes nothing."""
+    q, X, Y, Z = 19, 32, 12, 10
+    _, pb = fused_bc(q, True, lid=(0.0, 0.0, 0.0))
+    rng = np.random.default_rng(8)
+    mask = (rng.random((Z, Y, X)) < 0.1).astype(np.uint8)
+    with P.State(q, X, Y, Z, variant, P.FUSED) as a, P.State(q, X, Y, Z, variant, P.FUSED) as b:
+        a.set_tuning(P.capi.TUNE_SCHEME, 2)
+        b.set_tuning(P.capi.TUNE_SCHEME, 1)
+        for st in (a, b):
+            st.set_boundary(pb, mask)
+            st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 21, 1e-3), pb)
+            st.configure(0.58, (1e-5, 0, 0), pb)
+        for chunk in (1, 7, 2, 10):
+            a.step(chunk)
+            b.step(chunk)
+            assert np.array_equal(a.download_raw(), b.download_raw())
+            ra, ua = a.moments()
+            rb, ub = b.moments()
+            assert np.array_equal(ra, rb) and np.array_equal(ua, ub)
+        a.step_graph(8, 4)
+        b.step(8)
+        assert np.array_equal(a.download_raw(), b.download_raw())
 
 
 @pytest.mark.parametrize("variant", ["f64", "f32s"])
