@@ -1,0 +1,276 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X lattice-Boltzmann BGK step (one JSON line).
+
+Workload (the headline configuration): D3Q19, 256^3 lattice, walls on the two
+faces normal to the wall axis (the reference's up/down faces), periodic in the
+other two axes, seeded Bernoulli(0.10) interior solid voxels with halfway
+bounce-back, tau = 0.55, body force 5e-7 along x, FP32 shifted-population
+storage (f - w_k) with FP32 collision arithmetic.  A step is one fused
+collide + stream + bounce-back launch over the whole lattice.
+
+Reported:
+  value        MLUPS over every lattice cell (walls and solids included, as the
+               reference counts its loop), host wall clock around K steps;
+  roofline     algorithmic HBM bytes per launch / average launch duration
+               (HIP events on the state's stream), against 8 TB/s;
+  variants     the same workload with FP64, FP32 and FP16-shifted storage;
+  errors       velocity / density L2 error of each variant against the FP64
+               run after --error-steps steps;
+  cpu_baseline the reference's FP32 staged CPU step (restated with OpenMP and
+               AVX2) on this host's cores, bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--error-steps S]
+                       [--no-errors] [--no-cpu] [--no-variants] [--profile]
+                       [--copy-only]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS + % HBM roofline, D3Q19 256³: FP32/FP16 vs FP64 GPU at stated error"
+HBM_PEAK_GBS = 8000.0
+N = 256
+Q = 19
+TAU = 0.55
+FORCE = (5e-7, 0.0, 0.0)
+SOLID_FRACTION = 0.10
+SOLID_SEED = 20250303
+STORAGE_BYTES = {"f64": 8, "f32": 4, "f32s": 4, "f16s": 2, "f64a32": 8}
+DTYPE = {"f64": "f64", "f32": "f32", "f32s": "f32", "f16s": "f32", "f64a32": "f32"}
+
+
+def solid_mask():
+    rng = np.random.default_rng(SOLID_SEED)
+    return (rng.random((N, N, N)) < SOLID_FRACTION).astype(np.uint8)
+
+
+def make_state(P, variant, mask):
+    bc = P.channel()  # periodic east/west + north/south, no-slip up/down
+    st = P.State(Q, N, N, N, variant, P.FUSED)
+    st.set_boundary(bc, mask)
+    st.initialize(1.0, (0.0, 0.0, 0.0), None, bc)
+    st.configure(TAU, FORCE, bc)
+    return st
+
+
+def fluid_cells(mask):
+    m = mask.copy()
+    m[0] = 1
+    m[-1] = 1  # wall layers
+    return int((m == 0).sum())
+
+
+def algorithmic_bytes(variant, nfluid):
+    """2*Q*sizeof(storage) per fluid update plus the 1-byte flag per cell."""
+    return nfluid * 2 * Q * STORAGE_BYTES[variant] + N ** 3 * 1
+
+
+def time_variant(P, variant, mask, steps, warmup, graph):
+    st = make_state(P, variant, mask)
+    st.step(warmup)
+    st.synchronize()
+    spg = 10 if graph else 0
+    if graph:
+        st.time_steps(spg, spg)  # build the graph outside the timed region
+    t0 = time.perf_counter()
+    ev_ms = st.time_steps(steps, spg)
+    st.synchronize()
+    wall = time.perf_counter() - t0
+    scheme = st.scheme()
+    st.close()
+    return wall, ev_ms, scheme
+
+
+def errors_vs_f64(P, variants, mask, steps):
+    ref = make_state(P, "f64", mask)
+    ref.step(steps)
+    rho_r, u_r = ref.moments()
+    gm_r = ref.global_moments()
+    ref.close()
+    fluid = mask == 0
+    fluid[0] = False
+    fluid[-1] = False
+    out = {}
+    for v in variants:
+        st = make_state(P, v, mask)
+        gm0 = st.global_moments()
+        st.step(steps)
+        rho, u = st.moments()
+        gm = st.global_moments()
+        st.close()
+        du = u[fluid] - u_r[fluid]
+        dr = rho[fluid] - rho_r[fluid]
+        out[v] = {
+            "steps": steps,
+            "u_l2_rel": float(np.sqrt((du ** 2).sum() / (u_r[fluid] ** 2).sum())),
+            "u_max_abs": float(np.abs(du).max()),
+            "rho_l2_rel": float(np.sqrt((dr ** 2).sum() / (rho_r[fluid] ** 2).sum())),
+            "rho_dev_l2_rel": float(np.sqrt((dr ** 2).sum() / ((rho_r[fluid] - 1.0) ** 2).sum())),
+            "mass_drift_fp64": float((gm[0] - gm0[0]) / gm0[0]),
+        }
+    out["f64_reference"] = {"u_max": float(np.abs(u_r[fluid]).max()),
+                            "mass": float(gm_r[0]), "momentum_x": float(gm_r[1])}
+    return out
+
+
+def cpu_baseline(sample_steps=20, single_core_steps=4):
+    from oracle.pyoracle import CpuBaseline  # the checker side: restated reference CPU path
+    cb = CpuBaseline()
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    nu = (TAU - 0.5) / 3.0
+    t_all, ph_all = cb.run(Q, N, N, N, nu, FORCE, 2, sample_steps, cores)
+    t_one, _ = cb.run(Q, N, N, N, nu, FORCE, 1, single_core_steps, 1)
+    mlups_all = N ** 3 * sample_steps / t_all / 1e6
+    mlups_one = N ** 3 * single_core_steps / t_one / 1e6
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": round(mlups_all, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
+        "sample": (f"D3Q19 {N}^3 channel (walls up/down, periodic x/y), the reference's staged FP32 "
+                   f"step (collide AVX2+FMA, push, no-slip, periodic, moments, reset; OpenMP over z), "
+                   f"{sample_steps} timed steps after 2 warm-up; no interior solids (the reference has none)"),
+        "single_core_mlups": round(mlups_one, 2), "single_core_steps": single_core_steps,
+        "phase_seconds": {k: round(v, 4) for k, v in zip(
+            ["collide", "stream", "noslip", "periodic", "moments", "reset"], ph_all.tolist())},
+        "cpu_model": cpu_model,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--variant", default="f32s")
+    ap.add_argument("--error-steps", type=int, default=10000)
+    ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="timed steps only (for rocprofv3)")
+    ap.add_argument("--copy-only", action="store_true", help="run only the copy calibration kernel")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; every GPU box here has one, and the "
+              "multi-GPU slab decomposition is a design only (no N>1 figure).", file=sys.stderr)
+        sys.exit(2)
+
+    import plbpd_amd as P
+    from plbpd_amd import capi
+
+    if args.copy_only:
+        gbs, ms = capi.bench_copy(1 << 32, max(args.steps, 1))
+        print(json.dumps({"copy_gbps": gbs, "copy_ms": ms, "bytes": 1 << 32}))
+        return
+
+    mask = solid_mask()
+    nfluid = fluid_cells(mask)
+    print(f"[bench] D3Q19 {N}^3, {nfluid} fluid cells, variant {args.variant}", file=sys.stderr, flush=True)
+    wall, ev_ms, scheme = time_variant(P, args.variant, mask, args.steps, args.warmup, not args.no_graph)
+    ms_per_step = wall / args.steps * 1e3
+    mlups = N ** 3 * args.steps / wall / 1e6
+    kernel_ms = ev_ms / args.steps
+    bytes_launch = algorithmic_bytes(args.variant, nfluid)
+    achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_d3q19_256_{args.variant}.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    line = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.variant],
+        "data": "synthetic",
+        "config": {
+            "workload": (f"D3Q19 {N}^3 channel: walls on the up/down faces, periodic x and y, "
+                         f"{SOLID_FRACTION:.0%} Bernoulli solid voxels (seed {SOLID_SEED}), tau {TAU}, "
+                         f"gx {FORCE[0]}, {args.variant} storage, fused {scheme} step"),
+            "lattice": "D3Q19", "size": [N, N, N], "storage": args.variant, "scheme": scheme,
+            "tau": TAU, "force": list(FORCE), "solid_fraction": SOLID_FRACTION, "solid_seed": SOLID_SEED,
+            "fluid_cells": nfluid, "hip_graph": not args.no_graph,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "kernel_ms": round(kernel_ms, 5),
+        "fluid_mlups": round(nfluid * args.steps / wall / 1e6, 1),
+        "bytes_per_launch_algorithmic": bytes_launch,
+    }
+    if args.profile:
+        print(json.dumps(line))
+        return
+
+    gbs, _ = capi.bench_copy(1 << 32, 20)
+    line["copy_gbps"] = round(gbs, 1)
+    line["roofline"]["frac_of_copy"] = round(achieved / gbs, 4)
+    try:
+        line["device"] = P.device_info(0)
+    except Exception:
+        pass
+
+    if not args.no_variants:
+        var = {}
+        for v in ("f64", "f32", "f16s"):
+            if v == args.variant:
+                continue
+            w, e, sch = time_variant(P, v, mask, max(args.steps // 2, 50), args.warmup, not args.no_graph)
+            steps_v = max(args.steps // 2, 50)
+            kms = e / steps_v
+            ach = algorithmic_bytes(v, nfluid) / (kms * 1e-3) / 1e9
+            var[v] = {"mlups": round(N ** 3 * steps_v / w / 1e6, 1), "kernel_ms": round(kms, 5),
+                      "achieved_gbps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "scheme": sch}
+        var[args.variant] = {"mlups": line["value"], "kernel_ms": line["kernel_ms"],
+                             "achieved_gbps": line["roofline"]["achieved"], "frac": line["roofline"]["frac"],
+                             "scheme": scheme}
+        line["variants"] = var
+        if "f64" in var:
+            line["speedup_vs_f64"] = round(var[args.variant]["mlups"] / var["f64"]["mlups"], 3)
+
+    if not args.no_errors:
+        print(f"[bench] errors vs FP64 after {args.error_steps} steps", file=sys.stderr, flush=True)
+        line["errors_vs_f64"] = errors_vs_f64(P, ["f32s", "f32", "f64a32", "f16s"],
+                                              mask, args.error_steps)
+
+    if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        try:
+            line["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the checker library may be missing on a stripped tree
+            line["cpu_baseline"] = {"value": None, "unit": "MLUPS", "error": str(e)}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -212,6 +212,9 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
 #define PLB_SCHEME_PUSH 1   /*   collide, then scatter to neighbours          */
 #define PLB_SCHEME_PULL 2   /*   gather from neighbours, then collide         */
 int plb_set_tuning(plb_state* s, int key, int value);
+/* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
+ * step will use. */
+int plb_step_scheme(const plb_state* s, int* scheme);
 
 /* ---- batched domains (no reference counterpart: one domain per PLBPD,
  * PLBPD.hpp:14-17) --------------------------------------------------------- */

@@ -151,6 +151,7 @@ def lib():
     L.plb_time_steps.argtypes = [vp, i, i, C.POINTER(C.c_double)]
     L.plb_bench_copy.argtypes = [i, sz, i, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.plb_set_tuning.argtypes = [vp, i, i]
+    L.plb_step_scheme.argtypes = [vp, C.POINTER(C.c_int)]
     if hasattr(L, "plb_batch_create"):
         L.plb_batch_create.argtypes = [i, i, i, C.POINTER(DomainDesc), C.POINTER(BoundaryC), i, C.POINTER(vp)]
         L.plb_batch_destroy.argtypes = [vp]
@@ -317,6 +318,12 @@ class State:
 
     def set_tuning(self, key, value):
         check(lib().plb_set_tuning(self.h, int(key), int(value)))
+
+    def scheme(self):
+        """'push' or 'pull': the streaming scheme of the next fused step."""
+        v = C.c_int()
+        check(lib().plb_step_scheme(self.h, C.byref(v)))
+        return "pull" if v.value == SCHEME_PULL else "push"
 
     def global_moments(self):
         out = np.zeros(4, np.float64)

@@ -57,6 +57,13 @@ int plb_set_tuning(plb_state* s, int key, int value) {
     return PLB_OK;
 }
 
+int plb_step_scheme(const plb_state* s, int* scheme) {
+    if (!s || !scheme) return fail(PLB_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->mode != PLB_MODE_FUSED) return fail(PLB_ERR_LOGIC, "literal mode has no fused scheme");
+    *scheme = fused_uses_pull(s) ? PLB_SCHEME_PULL : PLB_SCHEME_PUSH;
+    return PLB_OK;
+}
+
 int plb_time_steps(plb_state* s, int nsteps, int steps_per_graph, double* ms) {
     if (!s || !ms) return fail(PLB_ERR_INVALID_ARGUMENT, "null argument");
     PLB_HIP_TRY(hipSetDevice(s->device));
