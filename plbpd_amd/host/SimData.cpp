@@ -1,0 +1,78 @@
+#include "SimData.hpp"
+
+#include <type_traits>
+
+#include "plbpd_state.hpp"
+
+namespace {
+constexpr int kDtype = std::is_same<plbpd::real, double>::value ? PLB_DTYPE_F64 : PLB_DTYPE_F32;
+}
+
+SimData::SimData(const std::tuple<size_t, size_t, size_t> domain_dimensions, const size_t num_directions)
+    : SimData(domain_dimensions, num_directions, plbpd::default_device_config()) {}
+
+SimData::SimData(const std::tuple<size_t, size_t, size_t> domain_dimensions, const size_t num_directions,
+                 const plbpd::DeviceConfig& cfg)
+    : rho(nullptr), u(nullptr), n(nullptr), ntmp(nullptr), dims_(domain_dimensions),
+      kdim_(num_directions), cfg_(cfg), state_(nullptr) {
+    bootstrap();
+}
+
+SimData::~SimData() {
+    delete rho;
+    delete u;
+    delete n;
+    delete ntmp;
+    if (state_) plb_state_destroy(state_);
+}
+
+void SimData::bootstrap() {
+    size_t X, Y, Z;
+    std::tie(X, Y, Z) = dims_;
+    rho = new Field::ScalarField<plbpd::real, 1>(Z, Y, X, 0);
+    u = new Field::VectorField<plbpd::real, 1>(Z, Y, X, 3, 0);
+    n = new Field::VectorField<plbpd::real, 1>(Z, Y, X, kdim_, 0);
+    ntmp = new Field::VectorField<plbpd::real, 1>(Z, Y, X, kdim_, 0);
+    plb_state_desc d;
+    d.q = (int32_t)kdim_;
+    d.x = (int32_t)X;
+    d.y = (int32_t)Y;
+    d.z = (int32_t)Z;
+    d.variant = cfg_.variant;
+    d.mode = cfg_.mode;
+    d.device = cfg_.device;
+    d.pop_pad = cfg_.pop_pad;
+    plbpd::check(plb_state_create(&d, &state_));
+}
+
+void SimData::sync_to_host() {
+    plbpd::check(plb_download(state_, PLB_FIELD_RHO, PLB_LAYOUT_REFERENCE, kDtype, rho->data()));
+    plbpd::check(plb_download(state_, PLB_FIELD_U, PLB_LAYOUT_REFERENCE, kDtype, u->data()));
+    plbpd::check(plb_download(state_, PLB_FIELD_N, PLB_LAYOUT_REFERENCE, kDtype, n->data()));
+    if (cfg_.mode == PLB_MODE_LITERAL)
+        plbpd::check(plb_download(state_, PLB_FIELD_NTMP, PLB_LAYOUT_REFERENCE, kDtype, ntmp->data()));
+}
+
+void SimData::sync_to_device() {
+    plbpd::check(plb_upload(state_, PLB_FIELD_N, PLB_LAYOUT_REFERENCE, kDtype, n->data()));
+    if (cfg_.mode == PLB_MODE_LITERAL) {
+        plbpd::check(plb_upload(state_, PLB_FIELD_RHO, PLB_LAYOUT_REFERENCE, kDtype, rho->data()));
+        plbpd::check(plb_upload(state_, PLB_FIELD_U, PLB_LAYOUT_REFERENCE, kDtype, u->data()));
+        plbpd::check(plb_upload(state_, PLB_FIELD_NTMP, PLB_LAYOUT_REFERENCE, kDtype, ntmp->data()));
+    }
+}
+
+void SimData::write_state(std::string dump_file) {
+    sync_to_host();
+    size_t X, Y, Z;
+    std::tie(X, Y, Z) = dims_;
+    plbpd::write_state_file(dump_file, X, Y, Z, kdim_, rho->data(), u->data(), n->data(),
+                            cfg_.mode == PLB_MODE_LITERAL ? ntmp->data() : nullptr);
+}
+
+void SimData::read_state(std::string dump_file) {
+    size_t X, Y, Z;
+    std::tie(X, Y, Z) = dims_;
+    plbpd::read_state_file(dump_file, X, Y, Z, kdim_, rho->data(), u->data(), n->data(), ntmp->data());
+    sync_to_device();
+}

@@ -1,0 +1,327 @@
+// Tests of the C++ drop-in classes (plbpd_amd/host), written like the
+// reference's src/utest mains but with checks that stay on in release builds.
+//   test_dropin cpu   the reference's known answers (TestLBModel, TestField,
+//                     TestDomain, TestEqlbDist) and error types; no GPU
+//   test_dropin gpu   PLBPD::march_in_time through the drop-in classes on the
+//                     GPU, compared bit for bit with the CPU oracle, in literal
+//                     and fused mode; state files; error types of device calls
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "Boundary.hpp"
+#include "EqlbDist.hpp"
+#include "Field.hpp"
+#include "Flow.hpp"
+#include "PLBPD.hpp"
+
+// ---- the CPU oracle's C interface (oracle/plb_oracle.cpp) ----
+extern "C" {
+typedef struct {
+    int type[6];
+    int has_vel[6];
+    double vel[6][3];
+    double rho_s;
+} orc_bc;
+typedef struct {
+    int kind;
+    unsigned long long seed;
+    double amp;
+} orc_init;
+int orc_lit_init_f64(int q, int X, int Y, int Z, double rho0, const double* u0, const orc_init* in,
+                     const orc_bc* bc, double* rho, double* u, double* n, double* ntmp);
+int orc_lit_run_f64(int q, int X, int Y, int Z, double tau, const double* F, const orc_bc* bc, int nsteps,
+                    double* rho, double* u, double* bufA, double* bufB);
+int orc_cor_init(int variant, int q, int X, int Y, int Z, double rho0, const double* u0, const orc_init* in,
+                 void* f);
+int orc_cor_run(int variant, int q, int X, int Y, int Z, double tau, const double* F, const orc_bc* bc,
+                const uint8_t* solid, int nsteps, void* bufA, void* bufB);
+int orc_cor_moments(int variant, int q, int X, int Y, int Z, const orc_bc* bc, const uint8_t* solid,
+                    const void* f, double* rho, double* u);
+int orc_cor_flags(int q, int X, int Y, int Z, const orc_bc* bc, const uint8_t* solid, uint8_t* flags);
+}
+
+namespace {
+
+int g_fail = 0;
+#define EXPECT(cond)                                                              \
+    do {                                                                          \
+        if (!(cond)) {                                                            \
+            std::fprintf(stderr, "FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);  \
+            ++g_fail;                                                             \
+        }                                                                         \
+    } while (0)
+
+template <class E>
+bool throws(const std::function<void()>& fn) {
+    try {
+        fn();
+    } catch (const E&) {
+        return true;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------------ CPU ---
+void test_lbmodel() {  // TestLBModel.cpp:9-166
+    struct Case {
+        const char* name;
+        uint32_t q;
+        std::vector<std::array<int32_t, 3> > c;
+        std::vector<float> w;
+        std::vector<uint32_t> rev;
+    };
+    const std::vector<Case> cases = {
+        {"D2Q9", 9,
+         {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}, {1, 0, 1}, {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}},
+         {4. / 9., 1. / 9., 1. / 9., 1. / 9., 1. / 9., 1. / 36., 1. / 36., 1. / 36., 1. / 36.},
+         {0, 2, 1, 4, 3, 6, 5, 8, 7}},
+        {"D3Q19", 19,
+         {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}, {1, 1, 0},
+          {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 0, 1}, {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {0, 1, 1},
+          {0, -1, -1}, {0, 1, -1}, {0, -1, 1}},
+         {1. / 3., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 36., 1. / 36., 1. / 36.,
+          1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.},
+         {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17}},
+    };
+    for (const Case& cs : cases) {
+        LBModel m(cs.name);
+        EXPECT(m.get_model_name() == cs.name);
+        EXPECT(m.get_num_directions() == cs.q);
+        EXPECT(m.get_speed_of_sound_squared() == 1.0f / 3.0f);
+        EXPECT(m.get_directional_velocities() == cs.c);
+        EXPECT(m.get_directional_weights() == cs.w);
+        EXPECT(m.get_reverse() == cs.rev);
+    }
+    LBModel m27("D3Q27");
+    EXPECT(m27.get_num_directions() == 27);
+    EXPECT((m27.get_directional_velocities()[26] == std::array<int32_t, 3>{1, -1, -1}));
+    EXPECT(m27.get_directional_weights()[19] == (float)(1. / 216.));
+    EXPECT(throws<std::logic_error>([] { LBModel m("D3Q15"); }));
+}
+
+void test_field() {  // TestField.cpp:9-175
+    Field::ScalarField<float, 0> sf(4, 5, 7, 0);
+    const Field::FieldExtents e = sf.get_extents();
+    EXPECT(e.zbegin == 0 && e.zend == 4 && e.ybegin == 0 && e.yend == 5 && e.xbegin == 0 && e.xend == 7);
+    uint32_t ctr = 0;
+    bool seq = true;
+    for (uint32_t z = e.zbegin; z < e.zend; ++z)
+        for (uint32_t y = e.ybegin; y < e.yend; ++y)
+            for (uint32_t x = e.xbegin; x < e.xend; ++x) seq &= sf.sub2ind(z, y, x) == ctr++;
+    EXPECT(seq);
+    Field::VectorField<float, 0> vf(4, 5, 7, 2, 0.0f);
+    EXPECT(vf.get_vector_length() == 2);
+    EXPECT(vf.sub2ind(1, 2, 3, 1) == ((1 * 5 + 2) * 7 + 3) * 2 + 1);
+    Field::ScalarField<float, 1> s1(2, 2, 3, 0.0f);
+    uint32_t zd, yd, xd;
+    std::tie(zd, yd, xd) = s1.get_dimensions();
+    EXPECT(zd == 4 && yd == 4 && xd == 5);
+    const Field::FieldExtents e1 = s1.get_extents();
+    EXPECT(e1.zbegin == 1 && e1.zend == 3 && e1.ybegin == 1 && e1.yend == 3 && e1.xbegin == 1 && e1.xend == 4);
+    for (uint32_t z = e1.zbegin; z < e1.zend; ++z)
+        for (uint32_t y = e1.ybegin; y < e1.yend; ++y)
+            for (uint32_t x = e1.xbegin; x < e1.xend; ++x) s1.at(z, y, x) = (float)s1.sub2ind(z, y, x);
+    EXPECT(s1.at(1, 2, 3) == 33);
+    EXPECT(s1.at(2, 1, 3) == 48);
+    EXPECT(throws<std::invalid_argument>([] { Field::ScalarField<float, 1> bad(0, 2, 2, 0.0f); }));
+    uint32_t nz, ny, nx;
+    std::tie(nz, ny, nx) = s1.get_neighbor(1, 1, 1, {1, -1, 1});
+    EXPECT(nz == 2 && ny == 0 && nx == 2);
+}
+
+void test_domain() {  // TestDomain.cpp:9-24
+    LBModel m("D3Q19");
+    const std::array<float, 3> vel = {-1148.14f, 762.878f, 3.2f};
+    const std::array<float, 3> force = {-0.145f, 0.0f, 4.2f};
+    Domain d(&m, 5, 4, 3, 0.5f, 1.0f, vel, force);
+    size_t x, y, z;
+    std::tie(x, y, z) = d.get_dimensions();
+    EXPECT(x == 5 && y == 4 && z == 3);
+    EXPECT(d.get_fluid_viscosity() == 0.5f);
+    EXPECT(d.get_fluid_density() == 1.0f);
+    EXPECT(d.get_init_flow_velocity() == vel);
+    EXPECT(d.get_external_force() == force);
+    EXPECT(d.get_relaxation_time() == (double)(3.0f * 0.5f + 0.5f));
+}
+
+void test_eqlbdist() {  // TestEqlbDist.cpp:11-35
+    LBModel m("D2Q9");
+    std::vector<float> n(9, 0.0f);
+    EqlbDist eq(&m);
+    eq(1.0f, {0.5f, 0.5f, -0.5f}, n);
+    const std::vector<float> expected = {-0.05555555597, 0.277777791, -0.05555555597, -0.05555555597, 0.277777791,
+                                         -0.003472222248, -0.003472222248, 0.2048611194, 0.03819444403};
+    float err = 0.0f;
+    for (int k = 0; k < 9; ++k) {
+        const float t = expected[k] - n[k];
+        err += t * t;
+    }
+    EXPECT(err == 0.0f);
+}
+
+void test_flow_names() {
+    EXPECT(throws<std::invalid_argument>([] { Flow f("Taylor"); }));
+    Flow f("Poiseuille2D");
+    EXPECT(f.get_num_timesteps() == 5000);
+    EXPECT(f.get_lbmodel()->get_num_directions() == 9);
+}
+
+// ------------------------------------------------------------------ GPU ---
+orc_bc to_orc(const plb_boundary& b) {
+    orc_bc o;
+    for (int f = 0; f < 6; ++f) {
+        o.type[f] = b.type[f];
+        o.has_vel[f] = b.has_vel[f];
+        for (int i = 0; i < 3; ++i) o.vel[f][i] = b.vel[f][i];
+    }
+    o.rho_s = b.rho_s;
+    return o;
+}
+
+struct Setup {
+    std::unique_ptr<LBModel> model;
+    std::unique_ptr<Domain> domain;
+    std::unique_ptr<Boundary> boundary;
+};
+
+Setup couette(size_t X, size_t Y, size_t Z, double tau, std::array<double, 3> F, float lid) {
+    Setup s;
+    s.model.reset(new LBModel("D3Q19"));
+    s.domain.reset(new Domain(s.model.get(), X, Y, Z, Domain::Relaxation{tau}, 1.0, {0.0, 0.0, 0.0}, F));
+    s.domain->set_initial_state({PLB_INIT_PERTURBED, 77, 1e-3});
+    BoundaryType type = {{"east", "periodic"}, {"west", "periodic"}, {"north", "periodic"},
+                         {"south", "periodic"}, {"up", "noslip"},     {"down", "noslip"}};
+    BoundaryVelocity vel = {{"up", {lid, 0.0f, 0.0f}}, {"down", {0.0f, 0.0f, 0.0f}}};
+    s.boundary.reset(new Boundary(s.model.get(), s.domain.get(), 1.0f, type, vel));
+    return s;
+}
+
+void test_literal_march_matches_oracle() {
+    const int X = 12, Y = 8, Z = 10, steps = 40;
+    const double tau = 0.7;
+    const std::array<double, 3> F = {1e-5, 0.0, 0.0};
+    Setup s = couette(X, Y, Z, tau, F, 0.05f);
+    plbpd::DeviceConfig cfg;
+    cfg.mode = PLB_MODE_LITERAL;
+    cfg.variant = PLB_F64;
+    SimData sim(s.domain->get_dimensions(), 19, cfg);
+    PLBPD p(s.model.get(), s.domain.get(), s.boundary.get());
+    p.initialize(sim);
+    p.march_in_time(steps, sim);
+    sim.sync_to_host();
+
+    const size_t cells = (size_t)(X + 2) * (Y + 2) * (Z + 2);
+    std::vector<double> rho(cells), u(3 * cells), a(19 * cells), b(19 * cells);
+    const orc_bc ob = to_orc(s.boundary->descriptor());
+    const orc_init in = {PLB_INIT_PERTURBED, 77, 1e-3};
+    const double u0[3] = {0, 0, 0};
+    orc_lit_init_f64(19, X, Y, Z, 1.0, u0, &in, &ob, rho.data(), u.data(), a.data(), b.data());
+    const int which = orc_lit_run_f64(19, X, Y, Z, tau, F.data(), &ob, steps, rho.data(), u.data(), a.data(), b.data());
+    const std::vector<double>& n = which == 0 ? a : b;
+    EXPECT(std::memcmp(rho.data(), sim.rho->data(), cells * sizeof(double)) == 0);
+    EXPECT(std::memcmp(u.data(), sim.u->data(), 3 * cells * sizeof(double)) == 0);
+    EXPECT(std::memcmp(n.data(), sim.n->data(), 19 * cells * sizeof(double)) == 0);
+}
+
+void test_fused_march_matches_oracle(int variant) {
+    const int X = 16, Y = 8, Z = 10, steps = 30;
+    const double tau = 0.65;
+    const std::array<double, 3> F = {2e-5, 0.0, 0.0};
+    Setup s = couette(X, Y, Z, tau, F, 0.0f);
+    plbpd::DeviceConfig cfg;
+    cfg.mode = PLB_MODE_FUSED;
+    cfg.variant = variant;
+    SimData sim(s.domain->get_dimensions(), 19, cfg);
+    PLBPD p(s.model.get(), s.domain.get(), s.boundary.get());
+    p.initialize(sim);
+    p.march_in_time(steps, sim);
+    sim.sync_to_host();
+
+    const size_t n = (size_t)X * Y * Z;
+    const size_t es = variant == PLB_F64 ? 8 : (variant == PLB_F16S ? 2 : 4);
+    std::vector<unsigned char> fa(19 * n * es), fb(19 * n * es);
+    const orc_bc ob = to_orc(s.boundary->descriptor());
+    const orc_init in = {PLB_INIT_PERTURBED, 77, 1e-3};
+    const double u0[3] = {0, 0, 0};
+    orc_cor_init(variant, 19, X, Y, Z, 1.0, u0, &in, fa.data());
+    const int which = orc_cor_run(variant, 19, X, Y, Z, tau, F.data(), &ob, nullptr, steps, fa.data(), fb.data());
+    std::vector<double> rho(n), u(3 * n);
+    orc_cor_moments(variant, 19, X, Y, Z, &ob, nullptr, which == 0 ? fa.data() : fb.data(), rho.data(), u.data());
+    std::vector<uint8_t> flags(n);
+    orc_cor_flags(19, X, Y, Z, &ob, nullptr, flags.data());
+    bool same = true;
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) {
+                const size_t c = ((size_t)z * Y + y) * X + x;
+                if (flags[c]) continue;
+                same &= sim.rho->at(z + 1, y + 1, x + 1) == rho[c];
+                for (int i = 0; i < 3; ++i) same &= sim.u->at(z + 1, y + 1, x + 1, i) == u[3 * c + i];
+            }
+    EXPECT(same);
+}
+
+void test_device_errors() {
+    Setup s = couette(6, 5, 5, 0.8, {0, 0, 0}, 0.0f);
+    plbpd::DeviceConfig cfg;
+    SimData sim(s.domain->get_dimensions(), 19, cfg);
+    Streaming st(s.model.get());
+    EXPECT(throws<std::logic_error>([&] { st(sim, "pull"); }));
+    EXPECT(throws<std::logic_error>([&] { st(sim, "sideways"); }));
+    BoundaryType type = {{"up", "noslip"}, {"down", "noslip"}};
+    BoundaryVelocity vel = {{"up", {0.0f, 0.0f, 0.0f}}};
+    Boundary bad(s.model.get(), s.domain.get(), 1.0f, type, vel);
+    EXPECT(throws<std::out_of_range>([&] { bad.apply_noslip(sim); }));
+    EXPECT(throws<std::invalid_argument>([&] { SimData z(std::make_tuple(0, 4, 4), 19, cfg); }));
+}
+
+void test_state_file_resume() {
+    const int X = 10, Y = 6, Z = 8;
+    Setup s = couette(X, Y, Z, 0.75, {1e-5, 0, 0}, 0.02f);
+    plbpd::DeviceConfig cfg;
+    SimData a(s.domain->get_dimensions(), 19, cfg), b(s.domain->get_dimensions(), 19, cfg);
+    PLBPD p(s.model.get(), s.domain.get(), s.boundary.get());
+    p.initialize(a);
+    p.march_in_time(15, a);
+    const std::string path = "/tmp/plbpd_dropin_state.h5";
+    a.write_state(path);
+    p.initialize(b);
+    b.read_state(path);  // rho, u, n and ntmp, exactly
+    p.march_in_time(10, a);
+    p.march_in_time(10, b);
+    a.sync_to_host();
+    b.sync_to_host();
+    EXPECT(std::memcmp(a.n->data(), b.n->data(), a.n->size() * sizeof(double)) == 0);
+    std::remove(path.c_str());
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const std::string what = argc > 1 ? argv[1] : "cpu";
+    if (what == "cpu" || what == "all") {
+        test_lbmodel();
+        test_field();
+        test_domain();
+        test_eqlbdist();
+        test_flow_names();
+    }
+    if (what == "gpu" || what == "all") {
+        test_literal_march_matches_oracle();
+        for (int v : {PLB_F64, PLB_F32S, PLB_F16S}) test_fused_march_matches_oracle(v);
+        test_device_errors();
+        test_state_file_resume();
+    }
+    std::printf("%s: %d failure(s)\n", what.c_str(), g_fail);
+    return g_fail ? 1 : 0;
+}

@@ -1,0 +1,37 @@
+"""The C++ drop-in classes (plbpd_amd/host) through tests/cpp/test_dropin and
+the plbpd_run runner."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_dropin")
+RUN = os.path.join(ROOT, "plbpd_amd", "lib", "plbpd_run")
+
+needs_bin = pytest.mark.skipif(not os.path.exists(BIN), reason="tests/cpp not built (run __graft_entry__.build())")
+
+
+@needs_bin
+def test_dropin_reference_known_answers():
+    r = subprocess.run([BIN, "cpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@needs_bin
+@pytest.mark.gpu
+def test_dropin_march_in_time_matches_oracle():
+    r = subprocess.run([BIN, "gpu"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args", [["Couette3D", "--steps", "20"],
+                                  ["Poiseuille2D", "--steps", "200", "--fused", "--variant", "f32s"]])
+def test_runner(tmp_path, args):
+    out = tmp_path / "state.h5"
+    r = subprocess.run([RUN, *args, "--out", str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Time taken by timeloop" in r.stdout
+    assert out.exists() and out.stat().st_size > 0
