@@ -211,6 +211,8 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
 #define PLB_SCHEME_AUTO 0   /*   pull unless a wall moves                     */
 #define PLB_SCHEME_PUSH 1   /*   collide, then scatter to neighbours          */
 #define PLB_SCHEME_PULL 2   /*   gather from neighbours, then collide         */
+#define PLB_TUNE_TILE_ORDER 4 /* rows in B x B blocks of the y-z plane (0 linear, -1 auto) */
+#define PLB_TUNE_VECTOR 5     /* 1: row-per-wavefront pull kernel (x = 128 or 256)      */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
  * step will use. */

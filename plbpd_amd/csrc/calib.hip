@@ -47,6 +47,12 @@ int plb_set_tuning(plb_state* s, int key, int value) {
             }
             s->tune_scheme = value;
             break;
+        case PLB_TUNE_TILE_ORDER:
+            if (value < -1 || value > 64 || (value > 0 && (value & (value - 1))))
+                return fail(PLB_ERR_INVALID_ARGUMENT, "tile order must be -1, 0 or a power of two up to 64");
+            s->tune_order = value;
+            break;
+        case PLB_TUNE_VECTOR: s->tune_vector = value != 0; break;
         default: return fail(PLB_ERR_INVALID_ARGUMENT, "unknown tuning key");
     }
     if (s->graph) {

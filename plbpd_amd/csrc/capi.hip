@@ -131,7 +131,7 @@ int plb_device_info(int device, char* name, size_t name_len, int* num_cu, size_t
     hipDeviceProp_t p;
     PLB_HIP_TRY(hipGetDeviceProperties(&p, device));
     if (name && name_len) {
-        std::strncpy(name, p.name, name_len - 1);
+        std::strncpy(name, p.name[0] ? p.name : p.gcnArchName, name_len - 1);
         name[name_len - 1] = 0;
     }
     if (num_cu) *num_cu = p.multiProcessorCount;
@@ -209,6 +209,8 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->tune_nt_load = 0;
     s->tune_nt_store = 0;
     s->tune_scheme = PLB_SCHEME_AUTO;
+    s->tune_order = -1;
+    s->tune_vector = 0;
     s->post_collision = false;
     for (double& v : s->stage_ms) v = 0.0;
 

@@ -25,6 +25,7 @@ struct FusedArgs {
     uint64_t pstride;
     uint32_t tx_shift, tx_mask;  // TX = 1 << tx_shift
     uint32_t ntx, nty, ntiles;
+    uint32_t order;             // 0: linear tile rows; B: B x B blocks of (y, z) rows
     int32_t nt_load, nt_store;  // non-temporal loads / stores
     T tau, om, omm, F0, F1, F2;
     FusedBC<T> bc;
@@ -50,15 +51,28 @@ struct Cell {
     bool valid;
 };
 
-// block -> tile -> (x, y, z) of this thread's cell
+// block -> tile -> (x, y, z) of this thread's cell.  Tile rows (ty, z) are
+// taken either in linear order or in square blocks of `order` x `order` rows
+// of the y-z plane, so that the rows a row exchanges populations with (y +-1,
+// z +-1) are worked on at about the same time, while their lines are in L2.
 template <class T>
 __device__ inline Cell cell_of(const FusedArgs<T>& a) {
     const uint32_t t = xcd_tile(blockIdx.x, a.ntiles);
     const uint32_t tx = t % a.ntx;
     const uint32_t rest = t / a.ntx;
-    const uint32_t ty = rest % a.nty;
+    uint32_t ty, tz;
+    if (a.order) {
+        const uint32_t bsz = a.order * a.order;
+        const uint32_t b = rest / bsz, within = rest % bsz;
+        const uint32_t nby = a.nty / a.order;
+        ty = (b % nby) * a.order + within % a.order;
+        tz = (b / nby) * a.order + within / a.order;
+    } else {
+        ty = rest % a.nty;
+        tz = rest / a.nty;
+    }
     Cell p;
-    p.z = rest / a.nty;
+    p.z = tz;
     p.x = (tx << a.tx_shift) + (threadIdx.x & a.tx_mask);
     p.y = ty * (kBlock >> a.tx_shift) + (threadIdx.x >> a.tx_shift);
     p.valid = p.x < a.X && p.y < a.Y;
@@ -256,6 +270,133 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
             __builtin_nontemporal_store(P::store(v), B + (uint64_t)k * a.pstride + c);
         else
             B[(uint64_t)k * a.pstride + c] = P::store(v);
+    }
+}
+
+// Row-per-wavefront pull step, for X == 64 * VEC: lane l owns cells
+// [VEC*l, VEC*l + VEC) of one x-row and moves them with one VEC-wide aligned
+// load and store per population.  The one value of an x-shifted direction
+// that belongs to the neighbouring lane arrives by a cross-lane rotate, which
+// also wraps the periodic x seam (the wavefront holds the whole row).  The
+// per-cell arithmetic is bgk(), exactly as in k_pull.
+template <int Q, int V, int VEC, int NB>
+__global__ __launch_bounds__(kBlock) void k_pull_vec(const typename Prec<V>::S* __restrict__ A,
+                                                     typename Prec<V>::S* __restrict__ B,
+                                                     const uint8_t* __restrict__ flags,
+                                                     const uint32_t* __restrict__ nbmask,
+                                                     FusedArgs<typename Prec<V>::T> a) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    using S = typename P::S;
+    typedef S SV __attribute__((ext_vector_type(VEC)));
+    constexpr uint32_t kRows = kBlock / 64;
+    const uint32_t t = xcd_tile(blockIdx.x, a.ntiles);
+    uint32_t ty, tz;
+    if (a.order) {
+        const uint32_t bsz = a.order * a.order;
+        const uint32_t b = t / bsz, within = t % bsz;
+        const uint32_t nby = a.nty / a.order;
+        ty = (b % nby) * a.order + within % a.order;
+        tz = (b / nby) * a.order + within / a.order;
+    } else {
+        ty = t % a.nty;
+        tz = t / a.nty;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t y = ty * kRows + (threadIdx.x >> 6);
+    if (y >= a.Y) return;  // whole wavefront
+    const uint32_t z = tz;
+    const uint32_t x0 = lane * VEC;
+    const uint64_t XY = (uint64_t)a.X * a.Y;
+    const uint64_t c0 = (uint64_t)z * XY + (uint64_t)y * a.X + x0;
+
+    uint32_t m[VEC];
+    bool ss[VEC];
+    bool any_fluid = false;
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+        if (NB) {
+            m[j] = nbmask[c0 + j];
+            ss[j] = m[j] & 1u;
+        } else {
+            m[j] = 0;
+            ss[j] = flags[c0 + j] != 0;
+        }
+        any_fluid |= !ss[j];
+    }
+    if (!__any(any_fluid)) return;  // a wall row
+
+    const uint32_t yo[3] = {y == 0 ? a.Y - 1 : y - 1, y, y + 1 == a.Y ? 0 : y + 1};
+    const uint32_t zo[3] = {z == 0 ? a.Z - 1 : z - 1, z, z + 1 == a.Z ? 0 : z + 1};
+    const bool wyr[3] = {a.bc.wall_lo[1] && yo[0] == 0, false, a.bc.wall_hi[1] && yo[2] == a.Y - 1};
+    const bool wzr[3] = {a.bc.wall_lo[2] && zo[0] == 0, false, a.bc.wall_hi[2] && zo[2] == a.Z - 1};
+    const int lane_prev = (int)((lane + 63u) & 63u), lane_next = (int)((lane + 1u) & 63u);
+
+    T f[Q][VEC];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
+        const uint64_t srow = (uint64_t)zo[1 - cz] * XY + (uint64_t)yo[1 - cy] * a.X;
+        const SV sv = *reinterpret_cast<const SV*>(A + (uint64_t)k * a.pstride + srow + x0);
+        T v[VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) v[j] = P::load(sv[j]);
+        T g[VEC];
+        if (cx > 0) {
+            g[0] = __shfl(v[VEC - 1], lane_prev);
+#pragma unroll
+            for (int j = 1; j < VEC; ++j) g[j] = v[j - 1];
+        } else if (cx < 0) {
+#pragma unroll
+            for (int j = 0; j < VEC - 1; ++j) g[j] = v[j + 1];
+            g[VEC - 1] = __shfl(v[0], lane_next);
+        } else {
+#pragma unroll
+            for (int j = 0; j < VEC; ++j) g[j] = v[j];
+        }
+        // bounce-back where the source cell x - c_k is solid
+        bool bb[VEC];
+        bool any_bb = false;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+            if (NB) {
+                bb[j] = ((m[j] >> rev_k<Q>(k)) & 1u) != 0;
+            } else {
+                const uint32_t x = x0 + j;
+                const uint32_t xs = cx > 0 ? (x == 0 ? a.X - 1 : x - 1) : (cx < 0 ? (x + 1 == a.X ? 0 : x + 1) : x);
+                bb[j] = wyr[1 - cy] || wzr[1 - cz] || (cx != 0 && a.bc.wall_lo[0] && xs == 0) ||
+                        (cx != 0 && a.bc.wall_hi[0] && xs == a.X - 1);
+            }
+            any_bb |= bb[j] && !ss[j];
+        }
+        if (__any(any_bb)) {
+            const SV ov = *reinterpret_cast<const SV*>(A + (uint64_t)rev_k<Q>(k) * a.pstride + c0);
+#pragma unroll
+            for (int j = 0; j < VEC; ++j)
+                if (bb[j]) g[j] = P::load(ov[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) f[k][j] = g[j];
+    }
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+        T fj[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) fj[k] = f[k][j];
+        bgk<Q, P>(fj, a, true);
+#pragma unroll
+        for (int k = 0; k < Q; ++k) f[k][j] = fj[k];
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        SV o;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) o[j] = P::store(ss[j] ? (T)0 : f[k][j]);
+        SV* dst = reinterpret_cast<SV*>(B + (uint64_t)k * a.pstride + c0);
+        if (a.nt_store)
+            __builtin_nontemporal_store(o, dst);
+        else
+            *dst = o;
     }
 }
 
@@ -599,15 +740,20 @@ enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly };
 // Launch geometry and arguments shared by every fused-layout kernel.
 int launch_kind(plb_state* s, KernelKind kind) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
+    // the row-per-wavefront pull kernel takes rows of 64 * VEC cells
+    int vec = (kind == kPull && s->tune_vector) ? (s->X == 256 ? 4 : (s->X == 128 ? 2 : 0)) : 0;
+    if (vec && s->pstride % (size_t)vec) vec = 0;  // VEC-wide accesses must stay aligned
     unsigned tx = 64;
-    if (s->tune_tx > 0)
+    if (vec)
+        tx = 256;  // one row per wavefront, kBlock / 64 rows per block
+    else if (s->tune_tx > 0)
         tx = (unsigned)s->tune_tx;
     else if (s->X % 256 == 0)
         tx = 256;
     else if (s->X % 128 == 0)
         tx = 128;
-    const unsigned ty = kBlock / tx;
-    const uint32_t ntx = (uint32_t)((s->X + tx - 1) / tx);
+    const unsigned ty = vec ? kBlock / 64 : kBlock / tx;
+    const uint32_t ntx = vec ? 1u : (uint32_t)((s->X + tx - 1) / tx);
     const uint32_t nty = (uint32_t)((s->Y + ty - 1) / ty);
     const uint64_t ntiles = (uint64_t)ntx * nty * s->Z;
     if (ntiles >= (1ull << 31)) return fail(PLB_ERR_INVALID_ARGUMENT, "domain too large for one launch");
@@ -627,6 +773,18 @@ int launch_kind(plb_state* s, KernelKind kind) {
         a.ntx = ntx;
         a.nty = nty;
         a.ntiles = (uint32_t)ntiles;
+        a.order = 0;
+        if (s->tune_order > 0) {
+            if (nty % (uint32_t)s->tune_order == 0 && s->Z % s->tune_order == 0) a.order = (uint32_t)s->tune_order;
+        } else if (s->tune_order < 0) {
+            // automatic: the largest of 16, 8, 4 that tiles the rows (16 measured best
+            // on MI355X for D3Q19 256^3 around solids; no effect without them)
+            for (uint32_t b : {16u, 8u, 4u})
+                if (nty % b == 0 && s->Z % b == 0) {
+                    a.order = b;
+                    break;
+                }
+        }
         a.nt_load = s->tune_nt_load;
         a.nt_store = s->tune_nt_store;
         a.tau = (T)s->tau;
@@ -653,8 +811,16 @@ int launch_kind(plb_state* s, KernelKind kind) {
                 else hipLaunchKernelGGL((k_push<Q, V, 0, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
                 break;
             case kPull:
-                if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-                else hipLaunchKernelGGL((k_pull<Q, V, 0, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                if (vec == 4) {
+                    if (nb) hipLaunchKernelGGL((k_pull_vec<Q, V, 4, 1>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    else hipLaunchKernelGGL((k_pull_vec<Q, V, 4, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                } else if (vec == 2) {
+                    if (nb) hipLaunchKernelGGL((k_pull_vec<Q, V, 2, 1>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    else hipLaunchKernelGGL((k_pull_vec<Q, V, 2, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                } else {
+                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    else hipLaunchKernelGGL((k_pull<Q, V, 0, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                }
                 break;
             case kGather:
                 if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);

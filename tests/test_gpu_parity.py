@@ -160,6 +160,11 @@ FUSED_CASES = [  # q, X, Y, Z, walls?, solids?
     (19, 12, 10, 8, False, False),
     (27, 16, 10, 9, True, False),
     (27, 10, 9, 8, True, True),
+    # rows of 64 * VEC cells take the row-per-wavefront pull kernel
+    (19, 256, 4, 6, True, True),
+    (19, 128, 5, 7, False, False),
+    (27, 128, 4, 5, True, True),
+    (9, 256, 1, 8, True, False),
 ]
 SCHEMES = {"push": 1, "pull": 2}
 
@@ -192,6 +197,7 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
     orc.cor_run(o, tau, F, ob, 25, mask)
     with P.State(q, X, Y, Z, variant, P.FUSED) as st:
         st.set_tuning(P.capi.TUNE_SCHEME, SCHEMES[scheme])
+        st.set_tuning(P.capi.TUNE_VECTOR, 1 if X in (128, 256) else 0)
         st.set_boundary(pb, mask)
         st.initialize(1.0, (0.01, 0, 0), init, pb)
         st.configure(tau, F, pb)
@@ -206,6 +212,33 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
     else:
         assert np.array_equal(raw[fluid], o["f"][fluid])
     assert np.array_equal(rho, orho) and np.array_equal(u, ou)
+
+
+@pytest.mark.parametrize("variant", ["f64", "f32", "f32s", "f16s"])
+@pytest.mark.parametrize("X", [128, 256])
+def test_closed_box_vector_pull(orc, variant, X):
+    """All six faces stationary walls: bounce-back on x-walls happens per
+    element inside the row-per-wavefront kernel."""
+    q, Y, Z = 19, 6, 5
+    types = [NOSLIP] * 6
+    vel = {f: (0.0, 0.0, 0.0) for f in range(6)}
+    ob, pb = both_bc(types, vel)
+    rng = np.random.default_rng(2)
+    mask = (rng.random((Z, Y, X)) < 0.05).astype(np.uint8)
+    for solids in (None, mask):
+        o = orc.cor_state(variant, q, X, Y, Z)
+        orc.cor_init(o, 1.0, (0.0, 0, 0), (P.INIT_PERTURBED, 6, 1e-3))
+        orc.cor_run(o, 0.7, (1e-5, 2e-6, 0), ob, 15, solids)
+        with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+            st.set_tuning(P.capi.TUNE_SCHEME, 2)
+            st.set_tuning(P.capi.TUNE_VECTOR, 1)
+            st.set_boundary(pb, solids)
+            st.initialize(1.0, (0.0, 0, 0), (P.INIT_PERTURBED, 6, 1e-3), pb)
+            st.configure(0.7, (1e-5, 2e-6, 0), pb)
+            st.step(15)
+            raw = st.download_raw()
+        fluid = orc.cor_flags(q, X, Y, Z, ob, solids) == 0
+        assert np.array_equal(raw[fluid], o["f"][fluid])
 
 
 @pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
