@@ -46,9 +46,11 @@ STORAGE_BYTES = {"f64": 8, "f32": 4, "f32s": 4, "f16s": 2, "f64a32": 8}
 DTYPE = {"f64": "f64", "f32": "f32", "f32s": "f32", "f16s": "f32", "f64a32": "f32"}
 
 
-def solid_mask():
+def solid_mask(fraction=SOLID_FRACTION):
+    if fraction <= 0:
+        return None
     rng = np.random.default_rng(SOLID_SEED)
-    return (rng.random((N, N, N)) < SOLID_FRACTION).astype(np.uint8)
+    return (rng.random((N, N, N)) < fraction).astype(np.uint8)
 
 
 def make_state(P, variant, mask):
@@ -61,7 +63,7 @@ def make_state(P, variant, mask):
 
 
 def fluid_cells(mask):
-    m = mask.copy()
+    m = np.zeros((N, N, N), np.uint8) if mask is None else mask.copy()
     m[0] = 1
     m[-1] = 1  # wall layers
     return int((m == 0).sum())
@@ -94,7 +96,7 @@ def errors_vs_f64(P, variants, mask, steps):
     rho_r, u_r = ref.moments()
     gm_r = ref.global_moments()
     ref.close()
-    fluid = mask == 0
+    fluid = (mask == 0) if mask is not None else np.ones((N, N, N), bool)
     fluid[0] = False
     fluid[-1] = False
     out = {}
@@ -155,6 +157,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--variant", default="f32s")
+    ap.add_argument("--solid-fraction", type=float, default=SOLID_FRACTION)
     ap.add_argument("--error-steps", type=int, default=10000)
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -177,7 +180,7 @@ def main():
         print(json.dumps({"copy_gbps": gbs, "copy_ms": ms, "bytes": 1 << 32}))
         return
 
-    mask = solid_mask()
+    mask = solid_mask(args.solid_fraction)
     nfluid = fluid_cells(mask)
     print(f"[bench] D3Q19 {N}^3, {nfluid} fluid cells, variant {args.variant}", file=sys.stderr, flush=True)
     wall, ev_ms, scheme = time_variant(P, args.variant, mask, args.steps, args.warmup, not args.no_graph)
@@ -188,7 +191,8 @@ def main():
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_d3q19_256_{args.variant}.json")
+    tag = f"d3q19_256_{args.variant}" + ("_solids" if mask is not None else "_walls")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{tag}.json")
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
@@ -210,10 +214,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"D3Q19 {N}^3 channel: walls on the up/down faces, periodic x and y, "
-                         f"{SOLID_FRACTION:.0%} Bernoulli solid voxels (seed {SOLID_SEED}), tau {TAU}, "
+                         f"{args.solid_fraction:.0%} Bernoulli solid voxels (seed {SOLID_SEED}), tau {TAU}, "
                          f"gx {FORCE[0]}, {args.variant} storage, fused {scheme} step"),
             "lattice": "D3Q19", "size": [N, N, N], "storage": args.variant, "scheme": scheme,
-            "tau": TAU, "force": list(FORCE), "solid_fraction": SOLID_FRACTION, "solid_seed": SOLID_SEED,
+            "tau": TAU, "force": list(FORCE), "solid_fraction": args.solid_fraction, "solid_seed": SOLID_SEED,
             "fluid_cells": nfluid, "hip_graph": not args.no_graph,
         },
         "roofline": {
