@@ -203,16 +203,27 @@ int plb_time_steps(plb_state* s, int nsteps, int steps_per_graph, double* ms);
 /* Copy-kernel calibration: copies `bytes` device-to-device `iters` times;
  * gbps counts bytes read plus bytes written. */
 int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms_per_copy);
-/* Fused-kernel tuning knobs. */
-#define PLB_TUNE_BLOCK_X 0  /* cells per block row: 64, 128, 256 (0 = from x) */
-#define PLB_TUNE_NT_LOAD 1  /* non-temporal population loads (0/1)            */
-#define PLB_TUNE_NT_STORE 2 /* non-temporal population stores (0/1)           */
-#define PLB_TUNE_SCHEME 3   /* streaming scheme of the fused step:            */
-#define PLB_SCHEME_AUTO 0   /*   pull unless a wall moves                     */
-#define PLB_SCHEME_PUSH 1   /*   collide, then scatter to neighbours          */
-#define PLB_SCHEME_PULL 2   /*   gather from neighbours, then collide         */
-#define PLB_TUNE_TILE_ORDER 4 /* rows in B x B blocks of the y-z plane (0 linear, -1 auto) */
-#define PLB_TUNE_VECTOR 5     /* 1: row-per-wavefront pull kernel (x = 128 or 256)      */
+/* Fused-kernel tuning knobs.  The defaults are the settings measured fastest
+ * on MI355X for D3Q19 256^3; a change drops a captured step graph. */
+#define PLB_TUNE_BLOCK_X 0  /* cells per block row: 64, 128, 256 (0 = from x)   */
+#define PLB_TUNE_NT_LOAD 1  /* non-temporal population loads of the cell-per-    */
+                            /* thread kernels (0/1; default 0).  The row pull    */
+                            /* kernel's aligned loads are always non-temporal.  */
+#define PLB_TUNE_NT_STORE 2 /* non-temporal population stores (0/1; default 1) */
+#define PLB_TUNE_SCHEME 3   /* streaming scheme of the fused step:              */
+#define PLB_SCHEME_AUTO 0   /*   pull unless a wall moves (default)             */
+#define PLB_SCHEME_PUSH 1   /*   collide, then scatter to neighbours            */
+#define PLB_SCHEME_PULL 2   /*   gather from neighbours, then collide           */
+#define PLB_TUNE_TILE_ORDER 4 /* rows in B x B blocks of the y-z plane (B a power */
+                              /* of two dividing the row counts; default 0 =      */
+                              /* linear)                                          */
+#define PLB_TUNE_VECTOR 5     /* row pull kernel with 1, 2 or 4 aligned cells per */
+                              /* lane (x = 64, 128 or 256); 0 = cell-per-thread   */
+                              /* pull kernel; -1 (default) = row kernel without   */
+                              /* interior solids (4 cells per lane for 2-byte     */
+                              /* storage, else 1), cell kernel around solids      */
+#define PLB_TUNE_XCD 6        /* 1: each XCD sweeps its own contiguous run of     */
+                              /* tiles; 0 (default): workgroup i takes tile i     */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
  * step will use. */

@@ -48,11 +48,16 @@ int plb_set_tuning(plb_state* s, int key, int value) {
             s->tune_scheme = value;
             break;
         case PLB_TUNE_TILE_ORDER:
-            if (value < -1 || value > 64 || (value > 0 && (value & (value - 1))))
-                return fail(PLB_ERR_INVALID_ARGUMENT, "tile order must be -1, 0 or a power of two up to 64");
+            if (value < 0 || value > 64 || (value > 0 && (value & (value - 1))))
+                return fail(PLB_ERR_INVALID_ARGUMENT, "tile order must be 0 or a power of two up to 64");
             s->tune_order = value;
             break;
-        case PLB_TUNE_VECTOR: s->tune_vector = value != 0; break;
+        case PLB_TUNE_XCD: s->tune_xcd = value != 0; break;
+        case PLB_TUNE_VECTOR:
+            if (value != -1 && value != 0 && value != 1 && value != 2 && value != 4)
+                return fail(PLB_ERR_INVALID_ARGUMENT, "row pull width must be -1, 0, 1, 2 or 4 cells per lane");
+            s->tune_vector = value;
+            break;
         default: return fail(PLB_ERR_INVALID_ARGUMENT, "unknown tuning key");
     }
     if (s->graph) {

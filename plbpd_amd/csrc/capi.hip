@@ -206,11 +206,16 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->graph_parity = 0;
     s->timers_on = false;
     s->tune_tx = 0;
+    // defaults measured fastest on MI355X for D3Q19 256^3 (FP64, FP32, FP16
+    // storage, with and without interior solids): plain loads, non-temporal
+    // stores, workgroup i on tile i in linear row order, the row pull kernel
+    // where there are no interior solids
     s->tune_nt_load = 0;
-    s->tune_nt_store = 0;
+    s->tune_nt_store = 1;
     s->tune_scheme = PLB_SCHEME_AUTO;
-    s->tune_order = -1;
-    s->tune_vector = 0;
+    s->tune_order = 0;
+    s->tune_vector = -1;
+    s->tune_xcd = 0;
     s->post_collision = false;
     for (double& v : s->stage_ms) v = 0.0;
 

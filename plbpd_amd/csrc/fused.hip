@@ -15,109 +15,11 @@
 #include <type_traits>
 #include <vector>
 
-#include "state.hpp"
+#include "fused_kernels.hpp"
 
 namespace plb {
 
-template <class T>
-struct FusedArgs {
-    uint32_t X, Y, Z;
-    uint64_t pstride;
-    uint32_t tx_shift, tx_mask;  // TX = 1 << tx_shift
-    uint32_t ntx, nty, ntiles;
-    uint32_t order;             // 0: linear tile rows; B: B x B blocks of (y, z) rows
-    int32_t nt_load, nt_store;  // non-temporal loads / stores
-    T tau, om, omm, F0, F1, F2;
-    FusedBC<T> bc;
-};
-
 namespace {
-
-constexpr unsigned kBlock = 256;
-
-template <int Q>
-__host__ __device__ constexpr int rev_k(int k) { return rev<Q>(k); }
-
-// Workgroup i runs on XCD i % 8; give each XCD a contiguous run of tiles so
-// that the rows a cell pushes into are mostly resident in the same L2.
-__device__ inline uint32_t xcd_tile(uint32_t i, uint32_t n) {
-    const uint32_t n8 = n >> 3, r = n & 7u, j = i & 7u;
-    return j * n8 + (j < r ? j : r) + (i >> 3);
-}
-
-struct Cell {
-    uint32_t x, y, z;
-    uint64_t c;
-    bool valid;
-};
-
-// block -> tile -> (x, y, z) of this thread's cell.  Tile rows (ty, z) are
-// taken either in linear order or in square blocks of `order` x `order` rows
-// of the y-z plane, so that the rows a row exchanges populations with (y +-1,
-// z +-1) are worked on at about the same time, while their lines are in L2.
-template <class T>
-__device__ inline Cell cell_of(const FusedArgs<T>& a) {
-    const uint32_t t = xcd_tile(blockIdx.x, a.ntiles);
-    const uint32_t tx = t % a.ntx;
-    const uint32_t rest = t / a.ntx;
-    uint32_t ty, tz;
-    if (a.order) {
-        const uint32_t bsz = a.order * a.order;
-        const uint32_t b = rest / bsz, within = rest % bsz;
-        const uint32_t nby = a.nty / a.order;
-        ty = (b % nby) * a.order + within % a.order;
-        tz = (b / nby) * a.order + within / a.order;
-    } else {
-        ty = rest % a.nty;
-        tz = rest / a.nty;
-    }
-    Cell p;
-    p.z = tz;
-    p.x = (tx << a.tx_shift) + (threadIdx.x & a.tx_mask);
-    p.y = ty * (kBlock >> a.tx_shift) + (threadIdx.x >> a.tx_shift);
-    p.valid = p.x < a.X && p.y < a.Y;
-    p.c = (uint64_t)p.z * a.X * a.Y + (uint64_t)p.y * a.X + p.x;
-    return p;
-}
-
-// Moments (CalcMoments.hpp:27-35 order) and, when `collide`, the BGK update
-// with the velocity-shift force (CollisionSRT.cpp:72-91, 170-175) in place.
-// Returns rho.  Shifted storage sums deviations and adds 1 afterwards.
-template <int Q, class P>
-__device__ inline typename P::T bgk(typename P::T (&f)[Q], const FusedArgs<typename P::T>& a, bool collide) {
-    using T = typename P::T;
-    T r = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        r += f[k];
-        j0 += f[k] * (T)Lat<Q>::cx(k);
-        j1 += f[k] * (T)Lat<Q>::cy(k);
-        j2 += f[k] * (T)Lat<Q>::cz(k);
-    }
-    const T drho = r;
-    if (P::shifted) r = (T)1 + r;
-    if (!collide) return r;
-    const T ri = (T)1 / r;
-    const T tri = a.tau / r;
-    const T u0 = j0 * ri + a.F0 * tri;
-    const T u1 = j1 * ri + a.F1 * tri;
-    const T u2 = j2 * ri + a.F2 * tri;
-    const T usq = u0 * u0 + u1 * u1 + u2 * u2;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const T cu = (k == 0) ? (T)0
-                              : (T)Lat<Q>::cx(k) * u0 + (T)Lat<Q>::cy(k) * u1 + (T)Lat<Q>::cz(k) * u2;
-        const T w = (T)Lat<Q>::w(k);
-        if (P::shifted) {
-            const T geq = w * (drho + r * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
-            f[k] = a.omm * f[k] + a.om * geq;
-        } else {
-            const T neq = (w * r) * ((((T)1 + (T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq);
-            f[k] = a.omm * f[k] + a.om * neq;
-        }
-    }
-    return r;
-}
 
 template <class T>
 struct Nbr {
@@ -220,6 +122,96 @@ __global__ __launch_bounds__(kBlock) void k_push(const typename Prec<V>::S* __re
     }
 }
 
+// Element offsets from a cell to its x-1, x+1, y-1, y+1, z-1, z+1
+// neighbours with the periodic wrap folded in, in the unsigned index type I
+// (modular arithmetic: a "negative" offset wraps and the sum lands in range),
+// and whether each of those neighbour layers is a wall layer (a walled axis
+// never wraps for a fluid cell: the wall layer sits between it and the edge).
+template <class I>
+struct Seams {
+    I xm, xp, ym, yp, zm, zp;
+    bool wxm, wxp, wym, wyp, wzm, wzp;
+};
+
+template <class I, class T>
+__device__ inline Seams<I> seams(uint32_t x, uint32_t y, uint32_t z, const FusedArgs<T>& a) {
+    const I X = a.X, XY = (I)a.X * a.Y;
+    Seams<I> s;
+    s.xm = x == 0 ? (I)(a.X - 1) : (I)0 - (I)1;
+    s.xp = x + 1 == a.X ? (I)0 - (I)(a.X - 1) : (I)1;
+    s.ym = y == 0 ? (I)(a.Y - 1) * X : (I)0 - X;
+    s.yp = y + 1 == a.Y ? (I)0 - (I)(a.Y - 1) * X : X;
+    s.zm = z == 0 ? (I)(a.Z - 1) * XY : (I)0 - XY;
+    s.zp = z + 1 == a.Z ? (I)0 - (I)(a.Z - 1) * XY : XY;
+    s.wxm = a.bc.wall_lo[0] && x == 1;
+    s.wxp = a.bc.wall_hi[0] && x + 2 == a.X;
+    s.wym = a.bc.wall_lo[1] && y == 1;
+    s.wyp = a.bc.wall_hi[1] && y + 2 == a.Y;
+    s.wzm = a.bc.wall_lo[2] && z == 1;
+    s.wzp = a.bc.wall_hi[2] && z + 2 == a.Z;
+    return s;
+}
+
+// Offset from a cell to x - c_k (the pull source), and whether that source is
+// a wall layer.  The direction components are compile-time constants, so each
+// is at most three adds and three ors.
+template <int Q, int K, class I>
+__device__ inline I pull_offset(const Seams<I>& s) {
+    constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
+    I d = 0;
+    if (cx > 0) d += s.xm;
+    if (cx < 0) d += s.xp;
+    if (cy > 0) d += s.ym;
+    if (cy < 0) d += s.yp;
+    if (cz > 0) d += s.zm;
+    if (cz < 0) d += s.zp;
+    return d;
+}
+template <int Q, int K, class I>
+__device__ inline bool pull_wall(const Seams<I>& s) {
+    constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
+    bool w = false;
+    if (cx > 0) w = w || s.wxm;
+    if (cx < 0) w = w || s.wxp;
+    if (cy > 0) w = w || s.wym;
+    if (cy < 0) w = w || s.wyp;
+    if (cz > 0) w = w || s.wzm;
+    if (cz < 0) w = w || s.wzp;
+    return w;
+}
+
+template <int Q, int K, int V, int NB, bool NT, class I>
+__device__ inline void pull_gather(typename Prec<V>::T (&f)[Q], const typename Prec<V>::S* __restrict__ A,
+                                   I c, I ps, uint32_t m, const Seams<I>& s) {
+    if constexpr (K < Q) {
+        using P = Prec<V>;
+        constexpr int R = rev_k<Q>(K);
+        // source x - c_k; it is solid when neighbour rev(k) is
+        const bool solid = NB ? ((m >> R) & 1u) != 0 : pull_wall<Q, K>(s);
+        const I moved = (I)K * ps + (c + pull_offset<Q, K>(s));
+        const I own = (I)R * ps + c;
+        // branch-free select (a conditional here becomes divergent branches)
+        const I src = moved ^ ((moved ^ own) & ((I)0 - (I)solid));
+        f[K] = NT ? P::load(__builtin_nontemporal_load(A + src)) : P::load(A[src]);
+        pull_gather<Q, K + 1, V, NB, NT>(f, A, c, ps, m, s);
+    }
+}
+
+template <int Q, int V, bool NT, class I>
+__device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Prec<V>::S* __restrict__ B, I c,
+                                 I ps, bool zero) {
+    using P = Prec<V>;
+    using T = typename P::T;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const typename P::S v = P::store(zero ? (T)0 : f[k]);
+        if (NT)
+            __builtin_nontemporal_store(v, B + ((I)k * ps + c));
+        else
+            B[(I)k * ps + c] = v;
+    }
+}
+
 // Pull step on post-collision storage (stationary walls): each fluid cell
 // gathers population k from x - c_k, or from its own reverse slot when that
 // cell is solid (the same halfway bounce-back as the push step), then
@@ -227,7 +219,12 @@ __global__ __launch_bounds__(kBlock) void k_push(const typename Prec<V>::S* __re
 // only its own row, so every cache line of B is written whole, by one
 // instruction, even around interior solids.  COLLIDE = false gathers only
 // (post-collision -> pre-collision populations).
-template <int Q, int V, int NB, bool COLLIDE>
+//
+// Addressing is kept to a few integer adds per population: the direction
+// offsets are per-cell constants (seams()), indices are 32-bit (I = uint32_t)
+// whenever Q * pstride fits, and loads are unconditional (every index is in
+// range; solid lanes discard what they load and store zeros).
+template <int Q, int V, int NB, bool COLLIDE, class I, int NTM>
 __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __restrict__ A,
                                                  typename Prec<V>::S* __restrict__ B,
                                                  const uint8_t* __restrict__ flags,
@@ -237,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
     using T = typename P::T;
     const Cell p = cell_of(a);
     if (!p.valid) return;
-    const uint64_t c = p.c;
+    const I c = (I)p.c;
     uint32_t m = 0;
     bool self_solid;
     if (NB) {
@@ -248,156 +245,12 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
     }
     if (__all(self_solid)) return;
 
-    const Nbr<T> nb = neighbours(p, a);
-    const uint64_t XY = (uint64_t)a.X * a.Y;
+    const Seams<I> sm = seams<I>(p.x, p.y, p.z, a);
+    const I ps = (I)a.pstride;
     T f[Q];
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
-        // source x - c_k; it is solid when neighbour rev(k) is
-        const uint64_t s = (uint64_t)nb.zo[1 - cz] * XY + (uint64_t)nb.yo[1 - cy] * a.X + nb.xo[1 - cx];
-        const bool solid = NB ? ((m >> rev_k<Q>(k)) & 1u) != 0 : (nb.wx[1 - cx] || nb.wy[1 - cy] || nb.wz[1 - cz]);
-        const uint64_t src = solid ? (uint64_t)rev_k<Q>(k) * a.pstride + c : (uint64_t)k * a.pstride + s;
-        T v = (T)0;
-        if (!self_solid) v = a.nt_load ? P::load(__builtin_nontemporal_load(A + src)) : P::load(A[src]);
-        f[k] = v;
-    }
+    pull_gather<Q, 0, V, NB, (NTM & 1) != 0>(f, A, c, ps, m, sm);
     if (COLLIDE) bgk<Q, P>(f, a, true);
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const T v = self_solid ? (T)0 : f[k];
-        if (a.nt_store)
-            __builtin_nontemporal_store(P::store(v), B + (uint64_t)k * a.pstride + c);
-        else
-            B[(uint64_t)k * a.pstride + c] = P::store(v);
-    }
-}
-
-// Row-per-wavefront pull step, for X == 64 * VEC: lane l owns cells
-// [VEC*l, VEC*l + VEC) of one x-row and moves them with one VEC-wide aligned
-// load and store per population.  The one value of an x-shifted direction
-// that belongs to the neighbouring lane arrives by a cross-lane rotate, which
-// also wraps the periodic x seam (the wavefront holds the whole row).  The
-// per-cell arithmetic is bgk(), exactly as in k_pull.
-template <int Q, int V, int VEC, int NB>
-__global__ __launch_bounds__(kBlock) void k_pull_vec(const typename Prec<V>::S* __restrict__ A,
-                                                     typename Prec<V>::S* __restrict__ B,
-                                                     const uint8_t* __restrict__ flags,
-                                                     const uint32_t* __restrict__ nbmask,
-                                                     FusedArgs<typename Prec<V>::T> a) {
-    using P = Prec<V>;
-    using T = typename P::T;
-    using S = typename P::S;
-    typedef S SV __attribute__((ext_vector_type(VEC)));
-    constexpr uint32_t kRows = kBlock / 64;
-    const uint32_t t = xcd_tile(blockIdx.x, a.ntiles);
-    uint32_t ty, tz;
-    if (a.order) {
-        const uint32_t bsz = a.order * a.order;
-        const uint32_t b = t / bsz, within = t % bsz;
-        const uint32_t nby = a.nty / a.order;
-        ty = (b % nby) * a.order + within % a.order;
-        tz = (b / nby) * a.order + within / a.order;
-    } else {
-        ty = t % a.nty;
-        tz = t / a.nty;
-    }
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t y = ty * kRows + (threadIdx.x >> 6);
-    if (y >= a.Y) return;  // whole wavefront
-    const uint32_t z = tz;
-    const uint32_t x0 = lane * VEC;
-    const uint64_t XY = (uint64_t)a.X * a.Y;
-    const uint64_t c0 = (uint64_t)z * XY + (uint64_t)y * a.X + x0;
-
-    uint32_t m[VEC];
-    bool ss[VEC];
-    bool any_fluid = false;
-#pragma unroll
-    for (int j = 0; j < VEC; ++j) {
-        if (NB) {
-            m[j] = nbmask[c0 + j];
-            ss[j] = m[j] & 1u;
-        } else {
-            m[j] = 0;
-            ss[j] = flags[c0 + j] != 0;
-        }
-        any_fluid |= !ss[j];
-    }
-    if (!__any(any_fluid)) return;  // a wall row
-
-    const uint32_t yo[3] = {y == 0 ? a.Y - 1 : y - 1, y, y + 1 == a.Y ? 0 : y + 1};
-    const uint32_t zo[3] = {z == 0 ? a.Z - 1 : z - 1, z, z + 1 == a.Z ? 0 : z + 1};
-    const bool wyr[3] = {a.bc.wall_lo[1] && yo[0] == 0, false, a.bc.wall_hi[1] && yo[2] == a.Y - 1};
-    const bool wzr[3] = {a.bc.wall_lo[2] && zo[0] == 0, false, a.bc.wall_hi[2] && zo[2] == a.Z - 1};
-    const int lane_prev = (int)((lane + 63u) & 63u), lane_next = (int)((lane + 1u) & 63u);
-
-    T f[Q][VEC];
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
-        const uint64_t srow = (uint64_t)zo[1 - cz] * XY + (uint64_t)yo[1 - cy] * a.X;
-        const SV sv = *reinterpret_cast<const SV*>(A + (uint64_t)k * a.pstride + srow + x0);
-        T v[VEC];
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) v[j] = P::load(sv[j]);
-        T g[VEC];
-        if (cx > 0) {
-            g[0] = __shfl(v[VEC - 1], lane_prev);
-#pragma unroll
-            for (int j = 1; j < VEC; ++j) g[j] = v[j - 1];
-        } else if (cx < 0) {
-#pragma unroll
-            for (int j = 0; j < VEC - 1; ++j) g[j] = v[j + 1];
-            g[VEC - 1] = __shfl(v[0], lane_next);
-        } else {
-#pragma unroll
-            for (int j = 0; j < VEC; ++j) g[j] = v[j];
-        }
-        // bounce-back where the source cell x - c_k is solid
-        bool bb[VEC];
-        bool any_bb = false;
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) {
-            if (NB) {
-                bb[j] = ((m[j] >> rev_k<Q>(k)) & 1u) != 0;
-            } else {
-                const uint32_t x = x0 + j;
-                const uint32_t xs = cx > 0 ? (x == 0 ? a.X - 1 : x - 1) : (cx < 0 ? (x + 1 == a.X ? 0 : x + 1) : x);
-                bb[j] = wyr[1 - cy] || wzr[1 - cz] || (cx != 0 && a.bc.wall_lo[0] && xs == 0) ||
-                        (cx != 0 && a.bc.wall_hi[0] && xs == a.X - 1);
-            }
-            any_bb |= bb[j] && !ss[j];
-        }
-        if (__any(any_bb)) {
-            const SV ov = *reinterpret_cast<const SV*>(A + (uint64_t)rev_k<Q>(k) * a.pstride + c0);
-#pragma unroll
-            for (int j = 0; j < VEC; ++j)
-                if (bb[j]) g[j] = P::load(ov[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) f[k][j] = g[j];
-    }
-#pragma unroll
-    for (int j = 0; j < VEC; ++j) {
-        T fj[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) fj[k] = f[k][j];
-        bgk<Q, P>(fj, a, true);
-#pragma unroll
-        for (int k = 0; k < Q; ++k) f[k][j] = fj[k];
-    }
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        SV o;
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) o[j] = P::store(ss[j] ? (T)0 : f[k][j]);
-        SV* dst = reinterpret_cast<SV*>(B + (uint64_t)k * a.pstride + c0);
-        if (a.nt_store)
-            __builtin_nontemporal_store(o, dst);
-        else
-            *dst = o;
-    }
+    own_store<Q, V, (NTM & 2) != 0>(f, B, c, ps, self_solid);
 }
 
 // Collision only, in place (pre-collision -> post-collision storage).
@@ -740,19 +593,26 @@ enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly };
 // Launch geometry and arguments shared by every fused-layout kernel.
 int launch_kind(plb_state* s, KernelKind kind) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
-    // the row-per-wavefront pull kernel takes rows of 64 * VEC cells
-    int vec = (kind == kPull && s->tune_vector) ? (s->X == 256 ? 4 : (s->X == 128 ? 2 : 0)) : 0;
-    if (vec && s->pstride % (size_t)vec) vec = 0;  // VEC-wide accesses must stay aligned
+    // the row pull kernel (fused_row.hip) takes rows of 64 * vec * (1, 2 or 4)
+    // cells.  Automatic choice, from measurement on MI355X (D3Q19 256^3): it
+    // wins without interior solids (4 cells per lane for 2-byte storage, else
+    // 1); around solids its bounce-back needs a second aligned load per
+    // direction and the cell-per-thread kernel is faster.
+    int vec = kind == kPull ? s->tune_vector : 0;
+    if (vec < 0) vec = s->has_solids ? 0 : (s->sbytes == 2 ? 4 : 1);
+    if (vec && s->pstride % (size_t)vec) vec = 0;  // vec-wide accesses must stay aligned
+    const int row_rows = vec ? pull_row_rows(vec, s->X) : 0;
+    if (!row_rows) vec = 0;
     unsigned tx = 64;
     if (vec)
-        tx = 256;  // one row per wavefront, kBlock / 64 rows per block
+        tx = 256;
     else if (s->tune_tx > 0)
         tx = (unsigned)s->tune_tx;
     else if (s->X % 256 == 0)
         tx = 256;
     else if (s->X % 128 == 0)
         tx = 128;
-    const unsigned ty = vec ? kBlock / 64 : kBlock / tx;
+    const unsigned ty = vec ? (unsigned)row_rows : kBlock / tx;
     const uint32_t ntx = vec ? 1u : (uint32_t)((s->X + tx - 1) / tx);
     const uint32_t nty = (uint32_t)((s->Y + ty - 1) / ty);
     const uint64_t ntiles = (uint64_t)ntx * nty * s->Z;
@@ -774,17 +634,9 @@ int launch_kind(plb_state* s, KernelKind kind) {
         a.nty = nty;
         a.ntiles = (uint32_t)ntiles;
         a.order = 0;
-        if (s->tune_order > 0) {
-            if (nty % (uint32_t)s->tune_order == 0 && s->Z % s->tune_order == 0) a.order = (uint32_t)s->tune_order;
-        } else if (s->tune_order < 0) {
-            // automatic: the largest of 16, 8, 4 that tiles the rows (16 measured best
-            // on MI355X for D3Q19 256^3 around solids; no effect without them)
-            for (uint32_t b : {16u, 8u, 4u})
-                if (nty % b == 0 && s->Z % b == 0) {
-                    a.order = b;
-                    break;
-                }
-        }
+        if (s->tune_order > 0 && nty % (uint32_t)s->tune_order == 0 && s->Z % s->tune_order == 0)
+            a.order = (uint32_t)s->tune_order;
+        a.xcd = s->tune_xcd ? 1u : 0u;
         a.nt_load = s->tune_nt_load;
         a.nt_store = s->tune_nt_store;
         a.tau = (T)s->tau;
@@ -801,6 +653,8 @@ int launch_kind(plb_state* s, KernelKind kind) {
         S* B = (S*)s->buf[s->cur ^ 1];
         const dim3 grid((unsigned)ntiles), block(kBlock);
         const bool nb = s->has_solids;
+        // 32-bit element indices when every population index fits
+        const bool idx32 = (uint64_t)Q * s->pstride <= 0xffffffffull;
         switch (kind) {
             case kPush:
                 if (nb) hipLaunchKernelGGL((k_push<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
@@ -811,20 +665,44 @@ int launch_kind(plb_state* s, KernelKind kind) {
                 else hipLaunchKernelGGL((k_push<Q, V, 0, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
                 break;
             case kPull:
-                if (vec == 4) {
-                    if (nb) hipLaunchKernelGGL((k_pull_vec<Q, V, 4, 1>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-                    else hipLaunchKernelGGL((k_pull_vec<Q, V, 4, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-                } else if (vec == 2) {
-                    if (nb) hipLaunchKernelGGL((k_pull_vec<Q, V, 2, 1>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-                    else hipLaunchKernelGGL((k_pull_vec<Q, V, 2, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                if (vec) {
+                    launch_pull_row<Q, V>(a, vec, a.nt_store != 0, nb, A, B, s->flags, s->nbmask, grid.x, s->stream);
+                } else if (idx32) {
+                    // non-temporal loads (bit 0) / stores (bit 1) on the hot path only
+                    const int ntm = (a.nt_load ? 1 : 0) | (a.nt_store ? 2 : 0);
+                    auto go = [&](auto nbc, auto ntc) {
+                        constexpr int NBv = decltype(nbc)::value, NTv = decltype(ntc)::value;
+                        hipLaunchKernelGGL((k_pull<Q, V, NBv, true, uint32_t, NTv>), grid, block, 0, s->stream, A, B,
+                                           s->flags, s->nbmask, a);
+                    };
+                    using I0 = std::integral_constant<int, 0>;
+                    using I1 = std::integral_constant<int, 1>;
+                    using I2 = std::integral_constant<int, 2>;
+                    using I3 = std::integral_constant<int, 3>;
+                    if (nb) {
+                        if (ntm == 0) go(I1{}, I0{});
+                        else if (ntm == 1) go(I1{}, I1{});
+                        else if (ntm == 2) go(I1{}, I2{});
+                        else go(I1{}, I3{});
+                    } else {
+                        if (ntm == 0) go(I0{}, I0{});
+                        else if (ntm == 1) go(I0{}, I1{});
+                        else if (ntm == 2) go(I0{}, I2{});
+                        else go(I0{}, I3{});
+                    }
                 } else {
-                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-                    else hipLaunchKernelGGL((k_pull<Q, V, 0, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, true, uint64_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    else hipLaunchKernelGGL((k_pull<Q, V, 0, true, uint64_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
                 }
                 break;
             case kGather:
-                if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-                else hipLaunchKernelGGL((k_pull<Q, V, 0, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                if (idx32) {
+                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, false, uint32_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    else hipLaunchKernelGGL((k_pull<Q, V, 0, false, uint32_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                } else {
+                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, false, uint64_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    else hipLaunchKernelGGL((k_pull<Q, V, 0, false, uint64_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                }
                 break;
             case kCollideOnly:
                 hipLaunchKernelGGL((k_collide_only<Q, V>), grid, block, 0, s->stream, (S*)s->buf[s->cur],
@@ -850,13 +728,9 @@ bool moving_walls(const plb_state* s) {
 
 bool fused_uses_pull(const plb_state* s) {
     if (s->tune_scheme == PLB_SCHEME_PUSH) return false;
-    if (moving_walls(s)) return false;  // the pull step has no moving-wall term
-    if (s->tune_scheme == PLB_SCHEME_PULL) return true;
-    // Automatic choice, from measurement on MI355X (D3Q19 256^3): the pull
-    // step wins for 4- and 8-byte storage (whole-row stores, no partial
-    // lines around solids); with 2-byte storage its x-shifted loads span two
-    // cache lines per wavefront and the push step is faster.
-    return s->sbytes >= 4;
+    // the pull step has no moving-wall term; otherwise it is the faster step
+    // for every storage type on MI355X (whole-line stores around solids)
+    return !moving_walls(s);
 }
 
 // The populations in buf[cur] are stored either before collision (what the

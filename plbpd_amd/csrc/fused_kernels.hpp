@@ -1,0 +1,127 @@
+// Device-side pieces shared by the fused-layout kernels (fused.hip and
+// fused_row.hip): launch arguments, the block -> cell mapping and the BGK
+// update.  Layout in HBM (structure of arrays, no halo):
+//   f[k * pstride + c],  c = (z*Y + y)*X + x,  x fastest.
+#pragma once
+
+#include "state.hpp"
+
+namespace plb {
+
+template <class T>
+struct FusedArgs {
+    uint32_t X, Y, Z;
+    uint64_t pstride;
+    uint32_t tx_shift, tx_mask;  // TX = 1 << tx_shift
+    uint32_t ntx, nty, ntiles;
+    uint32_t order;             // 0: linear tile rows; B: B x B blocks of (y, z) rows
+    uint32_t xcd;               // 1: each XCD takes a contiguous run of tiles; 0: block i -> tile i
+    int32_t nt_load, nt_store;  // non-temporal loads / stores
+    T tau, om, omm, F0, F1, F2;
+    FusedBC<T> bc;
+};
+
+namespace {
+
+constexpr unsigned kBlock = 256;
+
+template <int Q>
+__host__ __device__ constexpr int rev_k(int k) { return rev<Q>(k); }
+
+// Workgroup i runs on XCD i % 8.  With `xcd`, each XCD takes a contiguous
+// run of tiles, so that the rows a cell pushes into (or a bounce-back reads
+// again) are mostly resident in the same L2; without it block i is tile i and
+// the chip sweeps one contiguous window of memory.
+__device__ inline uint32_t xcd_tile(uint32_t i, uint32_t n, uint32_t xcd) {
+    if (!xcd) return i;
+    const uint32_t n8 = n >> 3, r = n & 7u, j = i & 7u;
+    return j * n8 + (j < r ? j : r) + (i >> 3);
+}
+
+struct Cell {
+    uint32_t x, y, z;
+    uint64_t c;
+    bool valid;
+};
+
+// block -> tile -> (x, y, z) of this thread's cell.  Tile rows (ty, z) are
+// taken either in linear order or in square blocks of `order` x `order` rows
+// of the y-z plane, so that the rows a row exchanges populations with (y +-1,
+// z +-1) are worked on at about the same time, while their lines are in L2.
+template <class T>
+__device__ inline Cell cell_of(const FusedArgs<T>& a) {
+    const uint32_t t = xcd_tile(blockIdx.x, a.ntiles, a.xcd);
+    const uint32_t tx = t % a.ntx;
+    const uint32_t rest = t / a.ntx;
+    uint32_t ty, tz;
+    if (a.order) {
+        const uint32_t bsz = a.order * a.order;
+        const uint32_t b = rest / bsz, within = rest % bsz;
+        const uint32_t nby = a.nty / a.order;
+        ty = (b % nby) * a.order + within % a.order;
+        tz = (b / nby) * a.order + within / a.order;
+    } else {
+        ty = rest % a.nty;
+        tz = rest / a.nty;
+    }
+    Cell p;
+    p.z = tz;
+    p.x = (tx << a.tx_shift) + (threadIdx.x & a.tx_mask);
+    p.y = ty * (kBlock >> a.tx_shift) + (threadIdx.x >> a.tx_shift);
+    p.valid = p.x < a.X && p.y < a.Y;
+    p.c = (uint64_t)p.z * a.X * a.Y + (uint64_t)p.y * a.X + p.x;
+    return p;
+}
+
+// Moments (CalcMoments.hpp:27-35 order) and, when `collide`, the BGK update
+// with the velocity-shift force (CollisionSRT.cpp:72-91, 170-175) in place.
+// Returns rho.  Shifted storage sums deviations and adds 1 afterwards.
+template <int Q, class P>
+__device__ inline typename P::T bgk(typename P::T (&f)[Q], const FusedArgs<typename P::T>& a, bool collide) {
+    using T = typename P::T;
+    T r = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        r += f[k];
+        j0 += f[k] * (T)Lat<Q>::cx(k);
+        j1 += f[k] * (T)Lat<Q>::cy(k);
+        j2 += f[k] * (T)Lat<Q>::cz(k);
+    }
+    const T drho = r;
+    if (P::shifted) r = (T)1 + r;
+    if (!collide) return r;
+    const T ri = (T)1 / r;
+    const T tri = a.tau / r;
+    const T u0 = j0 * ri + a.F0 * tri;
+    const T u1 = j1 * ri + a.F1 * tri;
+    const T u2 = j2 * ri + a.F2 * tri;
+    const T usq = u0 * u0 + u1 * u1 + u2 * u2;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const T cu = (k == 0) ? (T)0
+                              : (T)Lat<Q>::cx(k) * u0 + (T)Lat<Q>::cy(k) * u1 + (T)Lat<Q>::cz(k) * u2;
+        const T w = (T)Lat<Q>::w(k);
+        if (P::shifted) {
+            const T geq = w * (drho + r * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
+            f[k] = a.omm * f[k] + a.om * geq;
+        } else {
+            const T neq = (w * r) * ((((T)1 + (T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq);
+            f[k] = a.omm * f[k] + a.om * neq;
+        }
+    }
+    return r;
+}
+
+}  // namespace
+
+// Row pull step (fused_row.hip): rows of X == 64 * vec * wpr cells, vec cells
+// per lane, wpr wavefronts per row; nts = non-temporal stores.  Returns false
+// if no kernel exists for (vec, X).
+template <int Q, int V>
+bool launch_pull_row(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts, bool nb,
+                     const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
+                     const uint32_t* nbmask, unsigned nblocks, hipStream_t stream);
+// Rows per block of the row pull step for (vec, X), or 0 where it has no kernel.
+int pull_row_rows(int vec, int X);
+
+}  // namespace plb

@@ -1,0 +1,267 @@
+// Row pull step of the fused mode: the same step as k_pull in fused.hip
+// (gather from x - c_k with halfway bounce-back, BGK with the velocity-shift
+// force, whole-line stores of the cell's own slots; CollisionSRT.cpp:153-176,
+// Streaming.cpp:63-74, Boundary.cpp:189-221), with every population load
+// aligned.
+//
+// An x-row of X = 64 * VEC * WPR cells is cut into WPR wavefront segments;
+// lane l of segment s owns cells x0 .. x0 + VEC - 1, x0 = (64 s + l) VEC.
+// Each population row (y - c_y, z - c_z) is loaded unshifted, so every cache
+// line is read by exactly one wavefront (which is what lets these loads be
+// non-temporal).  The x shift is done in registers: inside a lane by moving
+// values between its VEC cells, across lanes by a rotate, and across
+// segments through LDS.  The segments of a row wrap around, which is the
+// periodic x seam; a walled x axis has solid wall cells there, whose values
+// bounce-back replaces.
+#include <type_traits>
+
+#include "fused_kernels.hpp"
+
+namespace plb {
+
+namespace {
+
+// Index of direction k among the directions with c_x != 0.
+template <int Q>
+__host__ __device__ constexpr int xslot(int k) {
+    int n = 0;
+    for (int j = 0; j < k; ++j) n += Lat<Q>::cx(j) != 0;
+    return n;
+}
+template <int Q>
+__host__ __device__ constexpr int nxdirs() { return xslot<Q>(Q); }
+
+template <class S, int VEC, bool NT>
+__device__ inline void vload(const S* p, S (&v)[VEC]) {
+    typedef S SV __attribute__((ext_vector_type(VEC)));
+    if constexpr (VEC == 1) {
+        v[0] = NT ? __builtin_nontemporal_load(p) : *p;
+    } else {
+        const SV* q = reinterpret_cast<const SV*>(p);
+        const SV w = NT ? __builtin_nontemporal_load(q) : *q;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) v[j] = w[j];
+    }
+}
+template <class S, int VEC, bool NT>
+__device__ inline void vstore(S* p, const S (&v)[VEC]) {
+    typedef S SV __attribute__((ext_vector_type(VEC)));
+    if constexpr (VEC == 1) {
+        if (NT)
+            __builtin_nontemporal_store(v[0], p);
+        else
+            *p = v[0];
+    } else {
+        SV w;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) w[j] = v[j];
+        if (NT)
+            __builtin_nontemporal_store(w, reinterpret_cast<SV*>(p));
+        else
+            *reinterpret_cast<SV*>(p) = w;
+    }
+}
+
+template <int Q, int V, int VEC, int WPR, int NB, bool NTS>
+__global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* __restrict__ A,
+                                                     typename Prec<V>::S* __restrict__ B,
+                                                     const uint8_t* __restrict__ flags,
+                                                     const uint32_t* __restrict__ nbmask,
+                                                     FusedArgs<typename Prec<V>::T> a) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    using S = typename P::S;
+    constexpr uint32_t kWaves = kBlock / 64;
+    constexpr uint32_t kRows = kWaves / WPR;
+    constexpr int NX = nxdirs<Q>();
+    // segment edges: first value (for c_x < 0) and last value (c_x > 0)
+    __shared__ T edge_first[kWaves][NX];
+    __shared__ T edge_last[kWaves][NX];
+
+    const uint32_t t = xcd_tile(blockIdx.x, a.ntiles, a.xcd);
+    uint32_t ty, tz;
+    if (a.order) {
+        const uint32_t bsz = a.order * a.order;
+        const uint32_t b = t / bsz, within = t % bsz;
+        const uint32_t nby = a.nty / a.order;
+        ty = (b % nby) * a.order + within % a.order;
+        tz = (b / nby) * a.order + within / a.order;
+    } else {
+        ty = t % a.nty;
+        tz = t / a.nty;
+    }
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t seg = wave % WPR, rib = wave / WPR;
+    const uint32_t y = ty * kRows + rib, z = tz;
+    const uint32_t x0 = (seg * 64u + lane) * VEC;
+    const uint64_t XY = (uint64_t)a.X * a.Y;
+    const uint64_t c0 = (uint64_t)z * XY + (uint64_t)y * a.X + x0;
+
+    // A wavefront takes part in the edge exchange (one barrier per block)
+    // whatever it has to do; it loads and stores only if its row exists and
+    // holds a fluid cell.
+    uint32_t m[VEC];
+    bool ss[VEC];
+    bool any_fluid = false;
+    const bool row_ok = y < a.Y;
+    if (row_ok) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+            if (NB) {
+                m[j] = nbmask[c0 + j];
+                ss[j] = m[j] & 1u;
+            } else {
+                m[j] = 0;
+                ss[j] = flags[c0 + j] != 0;
+            }
+            any_fluid |= !ss[j];
+        }
+    }
+    const bool active = row_ok && __any(any_fluid);
+
+    T f[Q][VEC];
+    if (active) {
+        const uint32_t yo[3] = {y == 0 ? a.Y - 1 : y - 1, y, y + 1 == a.Y ? 0 : y + 1};
+        const uint32_t zo[3] = {z == 0 ? a.Z - 1 : z - 1, z, z + 1 == a.Z ? 0 : z + 1};
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const int cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
+            const uint64_t srow = (uint64_t)zo[1 - cz] * XY + (uint64_t)yo[1 - cy] * a.X;
+            S v[VEC];
+            vload<S, VEC, true>(A + (uint64_t)k * a.pstride + srow + x0, v);
+#pragma unroll
+            for (int j = 0; j < VEC; ++j) f[k][j] = P::load(v[j]);
+        }
+        if (lane == 0 || lane == 63) {
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                if (Lat<Q>::cx(k) < 0 && lane == 0) edge_first[wave][xslot<Q>(k)] = f[k][0];
+                if (Lat<Q>::cx(k) > 0 && lane == 63) edge_last[wave][xslot<Q>(k)] = f[k][VEC - 1];
+            }
+        }
+    }
+    if (WPR > 1) __syncthreads();
+    if (!active) return;
+
+    const uint32_t wprev = rib * WPR + (seg + WPR - 1) % WPR;
+    const uint32_t wnext = rib * WPR + (seg + 1) % WPR;
+    const int lane_prev = (int)((lane + 63u) & 63u), lane_next = (int)((lane + 1u) & 63u);
+    const bool wyr[3] = {a.bc.wall_lo[1] && y == 1, false, a.bc.wall_hi[1] && y + 2 == a.Y};
+    const bool wzr[3] = {a.bc.wall_lo[2] && z == 1, false, a.bc.wall_hi[2] && z + 2 == a.Z};
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
+        T g[VEC];
+        if (cx > 0) {
+            // source x - 1: the previous cell of this lane, of the previous
+            // lane, or of the previous segment's last lane
+            T from = __shfl(f[k][VEC - 1], lane_prev);
+            if (WPR > 1 && lane == 0) from = edge_last[wprev][xslot<Q>(k)];
+            g[0] = from;
+#pragma unroll
+            for (int j = 1; j < VEC; ++j) g[j] = f[k][j - 1];
+        } else if (cx < 0) {
+            T from = __shfl(f[k][0], lane_next);
+            if (WPR > 1 && lane == 63) from = edge_first[wnext][xslot<Q>(k)];
+#pragma unroll
+            for (int j = 0; j < VEC - 1; ++j) g[j] = f[k][j + 1];
+            g[VEC - 1] = from;
+        } else {
+#pragma unroll
+            for (int j = 0; j < VEC; ++j) g[j] = f[k][j];
+        }
+        // bounce-back where the source cell x - c_k is solid
+        bool bb[VEC];
+        bool any_bb = false;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+            if (NB) {
+                bb[j] = ((m[j] >> rev_k<Q>(k)) & 1u) != 0;
+            } else {
+                const uint32_t x = x0 + j;
+                bb[j] = wyr[1 - cy] || wzr[1 - cz] || (cx > 0 && a.bc.wall_lo[0] && x == 1) ||
+                        (cx < 0 && a.bc.wall_hi[0] && x + 2 == a.X);
+            }
+            any_bb |= bb[j] && !ss[j];
+        }
+        if (__any(any_bb)) {
+            S ov[VEC];
+            vload<S, VEC, false>(A + (uint64_t)rev_k<Q>(k) * a.pstride + c0, ov);
+#pragma unroll
+            for (int j = 0; j < VEC; ++j)
+                if (bb[j]) g[j] = P::load(ov[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) f[k][j] = g[j];
+    }
+    // collide, cell by cell
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+        T fj[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) fj[k] = f[k][j];
+        bgk<Q, P>(fj, a, true);
+#pragma unroll
+        for (int k = 0; k < Q; ++k) f[k][j] = fj[k];
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        S o[VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) o[j] = P::store(ss[j] ? (T)0 : f[k][j]);
+        vstore<S, VEC, NTS>(B + (uint64_t)k * a.pstride + c0, o);
+    }
+}
+
+template <int Q, int V, int VEC, int WPR>
+void launch_one(const FusedArgs<typename Prec<V>::T>& a, bool nts, bool nb, const typename Prec<V>::S* A,
+                typename Prec<V>::S* B, const uint8_t* flags, const uint32_t* nbmask, unsigned nblocks,
+                hipStream_t stream) {
+    const dim3 grid(nblocks), block(kBlock);
+#define PLB_ROW(NBv, NTv)                                                                                   \
+    hipLaunchKernelGGL((k_pull_row<Q, V, VEC, WPR, NBv, NTv>), grid, block, 0, stream, A, B, flags, nbmask, a)
+    if (nb) {
+        if (nts) PLB_ROW(1, true);
+        else PLB_ROW(1, false);
+    } else {
+        if (nts) PLB_ROW(0, true);
+        else PLB_ROW(0, false);
+    }
+#undef PLB_ROW
+}
+
+}  // namespace
+
+template <int Q, int V>
+bool launch_pull_row(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts, bool nb,
+                     const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
+                     const uint32_t* nbmask, unsigned nblocks, hipStream_t stream) {
+    const uint32_t X = a.X;
+    if (vec == 1 && X == 64) launch_one<Q, V, 1, 1>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
+    else if (vec == 1 && X == 128) launch_one<Q, V, 1, 2>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
+    else if (vec == 1 && X == 256) launch_one<Q, V, 1, 4>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
+    else if (vec == 2 && X == 128) launch_one<Q, V, 2, 1>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
+    else if (vec == 2 && X == 256) launch_one<Q, V, 2, 2>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
+    else if (vec == 4 && X == 256) launch_one<Q, V, 4, 1>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
+    else return false;
+    return true;
+}
+
+// Rows per block for (vec, X), or 0 where launch_pull_row has no kernel.
+int pull_row_rows(int vec, int X) {
+    if (vec == 1 && (X == 64 || X == 128 || X == 256)) return 4 / (X / 64);
+    if (vec == 2 && (X == 128 || X == 256)) return 4 / (X / 128);
+    if (vec == 4 && X == 256) return 4;
+    return 0;
+}
+
+#define PLB_INST(Q, V)                                                                                      \
+    template bool launch_pull_row<Q, V>(const FusedArgs<typename Prec<V>::T>&, int, bool, bool,            \
+                                        const typename Prec<V>::S*, typename Prec<V>::S*, const uint8_t*,  \
+                                        const uint32_t*, unsigned, hipStream_t);
+PLB_INST(9, kF64) PLB_INST(9, kF32) PLB_INST(9, kF64A32) PLB_INST(9, kF32S) PLB_INST(9, kF16S)
+PLB_INST(19, kF64) PLB_INST(19, kF32) PLB_INST(19, kF64A32) PLB_INST(19, kF32S) PLB_INST(19, kF16S)
+PLB_INST(27, kF64) PLB_INST(27, kF32) PLB_INST(27, kF64A32) PLB_INST(27, kF32S) PLB_INST(27, kF16S)
+#undef PLB_INST
+
+}  // namespace plb

@@ -81,8 +81,9 @@ struct plb_state {
     int tune_tx;  // block width in x (64, 128, 256); 0 = pick from X
     int tune_nt_load, tune_nt_store;
     int tune_scheme;      // PLB_SCHEME_*
-    int tune_order;       // tile-row order: 0 linear, B: B x B (y, z) blocks; -1 automatic
-    int tune_vector;      // 1: row-per-wavefront pull kernel where X allows (off by default)
+    int tune_order;       // tile-row order: 0 linear, B: B x B (y, z) blocks
+    int tune_xcd;         // 1: XCD-contiguous tile runs; 0: block i -> tile i
+    int tune_vector;      // row pull kernel cells per lane (1, 2, 4) where X allows; 0 off; -1 auto
     bool post_collision;  // fused: buf[cur] holds post-collision populations (pull step)
     // timers
     bool timers_on;

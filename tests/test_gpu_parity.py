@@ -160,7 +160,7 @@ FUSED_CASES = [  # q, X, Y, Z, walls?, solids?
     (19, 12, 10, 8, False, False),
     (27, 16, 10, 9, True, False),
     (27, 10, 9, 8, True, True),
-    # rows of 64 * VEC cells take the row-per-wavefront pull kernel
+    # rows of 64, 128 or 256 cells also run the row pull kernel (one cell per lane)
     (19, 256, 4, 6, True, True),
     (19, 128, 5, 7, False, False),
     (27, 128, 4, 5, True, True),
@@ -215,10 +215,14 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
 
 
 @pytest.mark.parametrize("variant", ["f64", "f32", "f32s", "f16s"])
-@pytest.mark.parametrize("X", [128, 256])
-def test_closed_box_vector_pull(orc, variant, X):
+@pytest.mark.parametrize("X,vec", [(64, 1), (128, 1), (128, 2), (256, 1), (256, 2), (256, 4)])
+@pytest.mark.parametrize("nt", [0, 1])
+def test_closed_box_vector_pull(orc, variant, X, vec, nt):
     """All six faces stationary walls: bounce-back on x-walls happens per
-    element inside the row-per-wavefront kernel."""
+    element inside the row pull kernel, whose x shifts cross lanes and (for
+    rows of several wavefronts) segments; with and without non-temporal
+    loads and stores."""
+    print(f"\n[case] row pull {variant} X={X} vec={vec} nt={nt}", flush=True)
     q, Y, Z = 19, 6, 5
     types = [NOSLIP] * 6
     vel = {f: (0.0, 0.0, 0.0) for f in range(6)}
@@ -231,7 +235,9 @@ def test_closed_box_vector_pull(orc, variant, X):
         orc.cor_run(o, 0.7, (1e-5, 2e-6, 0), ob, 15, solids)
         with P.State(q, X, Y, Z, variant, P.FUSED) as st:
             st.set_tuning(P.capi.TUNE_SCHEME, 2)
-            st.set_tuning(P.capi.TUNE_VECTOR, 1)
+            st.set_tuning(P.capi.TUNE_VECTOR, vec)
+            st.set_tuning(P.capi.TUNE_NT_LOAD, nt)
+            st.set_tuning(P.capi.TUNE_NT_STORE, nt)
             st.set_boundary(pb, solids)
             st.initialize(1.0, (0.0, 0, 0), (P.INIT_PERTURBED, 6, 1e-3), pb)
             st.configure(0.7, (1e-5, 2e-6, 0), pb)
@@ -239,6 +245,43 @@ def test_closed_box_vector_pull(orc, variant, X):
             raw = st.download_raw()
         fluid = orc.cor_flags(q, X, Y, Z, ob, solids) == 0
         assert np.array_equal(raw[fluid], o["f"][fluid])
+
+
+@pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
+@pytest.mark.parametrize("solids", [False, True])
+def test_tuning_knobs_keep_results(orc, variant, solids):
+    """Every launch-shape knob (XCD mapping, tile order, block width,
+    non-temporal loads and stores, row or cell kernel, push or pull) gives the
+    oracle's populations bit for bit."""
+    q, X, Y, Z = 19, 128, 8, 8
+    ob, pb = fused_bc(q, True, lid=(0.0, 0.0, 0.0))
+    mask = None
+    if solids:
+        mask = (np.random.default_rng(4).random((Z, Y, X)) < 0.1).astype(np.uint8)
+    init = (P.INIT_PERTURBED, 17, 1e-3)
+    o = orc.cor_state(variant, q, X, Y, Z)
+    orc.cor_init(o, 1.0, (0, 0, 0), init)
+    orc.cor_run(o, 0.6, (1e-5, 0, 0), ob, 9, mask)
+    fluid = orc.cor_flags(q, X, Y, Z, ob, mask) == 0
+    T = P.capi
+    combos = [{}, {T.TUNE_XCD: 1}, {T.TUNE_TILE_ORDER: 4}, {T.TUNE_TILE_ORDER: 8, T.TUNE_XCD: 1},
+              {T.TUNE_BLOCK_X: 64}, {T.TUNE_NT_LOAD: 1, T.TUNE_NT_STORE: 0}, {T.TUNE_VECTOR: 0},
+              {T.TUNE_VECTOR: 1}, {T.TUNE_VECTOR: 2, T.TUNE_XCD: 1}, {T.TUNE_SCHEME: 1},
+              {T.TUNE_SCHEME: 1, T.TUNE_NT_STORE: 0, T.TUNE_XCD: 1}]
+    for knobs in combos:
+        print(f"\n[case] knobs {variant} solids={solids} {knobs}", flush=True)
+        with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+            for k, v in knobs.items():
+                st.set_tuning(k, v)
+            st.set_boundary(pb, mask)
+            st.initialize(1.0, (0, 0, 0), init, pb)
+            st.configure(0.6, (1e-5, 0, 0), pb)
+            st.step(9)
+            raw = st.download_raw()
+        if variant == "f16s":
+            assert np.array_equal(raw.view(np.uint16)[fluid], o["f"].view(np.uint16)[fluid]), knobs
+        else:
+            assert np.array_equal(raw[fluid], o["f"][fluid]), knobs
 
 
 @pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
