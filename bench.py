@@ -12,7 +12,12 @@ Reported:
   value        MLUPS over every lattice cell (walls and solids included, as the
                reference counts its loop), host wall clock around K steps;
   roofline     algorithmic HBM bytes per launch / average launch duration
-               (HIP events on the state's stream), against 8 TB/s;
+               (HIP events on the state's stream), against 8 TB/s.  Bytes per
+               lattice update = 2*Q*sizeof(storage) + 1 flag byte (153 B for
+               D3Q19 FP32), times every interior cell per launch; traffic =
+               measured HBM bytes per launch (rocprofv3 FETCH_SIZE +
+               WRITE_SIZE, calibrated on a copy kernel) where a profile of
+               this workload is in profiles/;
   variants     the same workload with FP64, FP32 and FP16-shifted storage;
   errors       velocity / density L2 error of each variant against the FP64
                run after --error-steps steps;
@@ -69,9 +74,21 @@ def fluid_cells(mask):
     return int((m == 0).sum())
 
 
-def algorithmic_bytes(variant, nfluid):
-    """2*Q*sizeof(storage) per fluid update plus the 1-byte flag per cell."""
-    return nfluid * 2 * Q * STORAGE_BYTES[variant] + N ** 3 * 1
+def bytes_per_update(variant):
+    """Bytes per lattice update: every population read once and written once,
+    plus the 1-byte cell flag."""
+    return 2 * Q * STORAGE_BYTES[variant] + 1
+
+
+def algorithmic_bytes(variant):
+    """Bytes per launch: one update of every interior cell, wall and solid
+    cells included (the reference collides and streams them all)."""
+    return N ** 3 * bytes_per_update(variant)
+
+
+def fluid_bytes(variant, nfluid):
+    """The same count over fluid cells only (the flag byte over all cells)."""
+    return nfluid * 2 * Q * STORAGE_BYTES[variant] + N ** 3
 
 
 def time_variant(P, variant, mask, steps, warmup, graph):
@@ -187,7 +204,7 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     mlups = N ** 3 * args.steps / wall / 1e6
     kernel_ms = ev_ms / args.steps
-    bytes_launch = algorithmic_bytes(args.variant, nfluid)
+    bytes_launch = algorithmic_bytes(args.variant)
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
 
     traffic = None
@@ -230,7 +247,9 @@ def main():
         },
         "kernel_ms": round(kernel_ms, 5),
         "fluid_mlups": round(nfluid * args.steps / wall / 1e6, 1),
+        "bytes_per_update": bytes_per_update(args.variant),
         "bytes_per_launch_algorithmic": bytes_launch,
+        "fluid_only_gbps": round(fluid_bytes(args.variant, nfluid) / (kernel_ms * 1e-3) / 1e9, 1),
     }
     if args.profile:
         print(json.dumps(line))
@@ -252,7 +271,7 @@ def main():
             w, e, sch = time_variant(P, v, mask, max(args.steps // 2, 50), args.warmup, not args.no_graph)
             steps_v = max(args.steps // 2, 50)
             kms = e / steps_v
-            ach = algorithmic_bytes(v, nfluid) / (kms * 1e-3) / 1e9
+            ach = algorithmic_bytes(v) / (kms * 1e-3) / 1e9
             var[v] = {"mlups": round(N ** 3 * steps_v / w / 1e6, 1), "kernel_ms": round(kms, 5),
                       "achieved_gbps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "scheme": sch}
         var[args.variant] = {"mlups": line["value"], "kernel_ms": line["kernel_ms"],

@@ -222,8 +222,11 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                               /* pull kernel; -1 (default) = row kernel without   */
                               /* interior solids (4 cells per lane for 2-byte     */
                               /* storage, else 1), cell kernel around solids      */
-#define PLB_TUNE_XCD 6        /* 1: each XCD sweeps its own contiguous run of     */
-                              /* tiles; 0 (default): workgroup i takes tile i     */
+#define PLB_TUNE_XCD 6        /* tile mapping: 0 workgroup i takes tile i; 1 each */
+                              /* XCD sweeps its own contiguous run of tiles; 2    */
+                              /* each XCD takes a y-stripe of every z-plane (rows */
+                              /* % 8 == 0, else 0); -1 (default) = 2 around       */
+                              /* interior solids, else 0                          */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
  * step will use. */

@@ -208,14 +208,14 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->tune_tx = 0;
     // defaults measured fastest on MI355X for D3Q19 256^3 (FP64, FP32, FP16
     // storage, with and without interior solids): plain loads, non-temporal
-    // stores, workgroup i on tile i in linear row order, the row pull kernel
-    // where there are no interior solids
+    // stores, linear row order, the row pull kernel and workgroup i on tile i
+    // where there are no interior solids, XCD y-stripes around solids
     s->tune_nt_load = 0;
     s->tune_nt_store = 1;
     s->tune_scheme = PLB_SCHEME_AUTO;
     s->tune_order = 0;
     s->tune_vector = -1;
-    s->tune_xcd = 0;
+    s->tune_xcd = -1;
     s->post_collision = false;
     for (double& v : s->stage_ms) v = 0.0;
 

@@ -15,7 +15,7 @@ struct FusedArgs {
     uint32_t tx_shift, tx_mask;  // TX = 1 << tx_shift
     uint32_t ntx, nty, ntiles;
     uint32_t order;             // 0: linear tile rows; B: B x B blocks of (y, z) rows
-    uint32_t xcd;               // 1: each XCD takes a contiguous run of tiles; 0: block i -> tile i
+    uint32_t xcd;               // tile mapping: 0 block i -> tile i; 1 XCD runs; 2 XCD y-stripes
     int32_t nt_load, nt_store;  // non-temporal loads / stores
     T tau, om, omm, F0, F1, F2;
     FusedBC<T> bc;
@@ -28,14 +28,28 @@ constexpr unsigned kBlock = 256;
 template <int Q>
 __host__ __device__ constexpr int rev_k(int k) { return rev<Q>(k); }
 
-// Workgroup i runs on XCD i % 8.  With `xcd`, each XCD takes a contiguous
-// run of tiles, so that the rows a cell pushes into (or a bounce-back reads
-// again) are mostly resident in the same L2; without it block i is tile i and
-// the chip sweeps one contiguous window of memory.
-__device__ inline uint32_t xcd_tile(uint32_t i, uint32_t n, uint32_t xcd) {
-    if (!xcd) return i;
-    const uint32_t n8 = n >> 3, r = n & 7u, j = i & 7u;
-    return j * n8 + (j < r ? j : r) + (i >> 3);
+// Workgroup i runs on XCD i % 8 (each XCD has its own L2).  Tile order:
+//   0: workgroup i takes tile i; the chip sweeps one contiguous window.
+//   1: each XCD takes a contiguous run of tiles (eight separate windows).
+//   2: each XCD takes a fixed stripe of nty / 8 tile rows in every z-plane,
+//      z-plane after z-plane: one window for the chip, while the rows a row
+//      exchanges populations with (y +-1, z +-1) are on the same XCD.
+//      Needs nty % 8 == 0 (the launcher falls back to 0 otherwise).
+template <class T>
+__device__ inline uint32_t tile_of_block(const FusedArgs<T>& a) {
+    const uint32_t i = blockIdx.x;
+    if (a.xcd == 1) {
+        const uint32_t n8 = a.ntiles >> 3, r = a.ntiles & 7u, j = i & 7u;
+        return j * n8 + (j < r ? j : r) + (i >> 3);
+    }
+    if (a.xcd == 2) {
+        const uint32_t j = i & 7u, m = i >> 3;
+        const uint32_t sy = a.nty >> 3, per = a.ntx * sy;
+        const uint32_t z = m / per, r = m % per;
+        const uint32_t ty = j * sy + r / a.ntx, tx = r % a.ntx;
+        return (z * a.nty + ty) * a.ntx + tx;
+    }
+    return i;
 }
 
 struct Cell {
@@ -50,7 +64,7 @@ struct Cell {
 // z +-1) are worked on at about the same time, while their lines are in L2.
 template <class T>
 __device__ inline Cell cell_of(const FusedArgs<T>& a) {
-    const uint32_t t = xcd_tile(blockIdx.x, a.ntiles, a.xcd);
+    const uint32_t t = tile_of_block(a);
     const uint32_t tx = t % a.ntx;
     const uint32_t rest = t / a.ntx;
     uint32_t ty, tz;

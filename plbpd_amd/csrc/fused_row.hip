@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
     __shared__ T edge_first[kWaves][NX];
     __shared__ T edge_last[kWaves][NX];
 
-    const uint32_t t = xcd_tile(blockIdx.x, a.ntiles, a.xcd);
+    const uint32_t t = tile_of_block(a);
     uint32_t ty, tz;
     if (a.order) {
         const uint32_t bsz = a.order * a.order;

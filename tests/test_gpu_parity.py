@@ -253,7 +253,7 @@ def test_tuning_knobs_keep_results(orc, variant, solids):
     """Every launch-shape knob (XCD mapping, tile order, block width,
     non-temporal loads and stores, row or cell kernel, push or pull) gives the
     oracle's populations bit for bit."""
-    q, X, Y, Z = 19, 128, 8, 8
+    q, X, Y, Z = 19, 128, 16, 8  # 8 tile rows: the XCD y-stripe mapping applies
     ob, pb = fused_bc(q, True, lid=(0.0, 0.0, 0.0))
     mask = None
     if solids:
@@ -264,7 +264,7 @@ def test_tuning_knobs_keep_results(orc, variant, solids):
     orc.cor_run(o, 0.6, (1e-5, 0, 0), ob, 9, mask)
     fluid = orc.cor_flags(q, X, Y, Z, ob, mask) == 0
     T = P.capi
-    combos = [{}, {T.TUNE_XCD: 1}, {T.TUNE_TILE_ORDER: 4}, {T.TUNE_TILE_ORDER: 8, T.TUNE_XCD: 1},
+    combos = [{}, {T.TUNE_XCD: 1}, {T.TUNE_XCD: 2}, {T.TUNE_XCD: 2, T.TUNE_VECTOR: 0}, {T.TUNE_TILE_ORDER: 4}, {T.TUNE_TILE_ORDER: 8, T.TUNE_XCD: 1},
               {T.TUNE_BLOCK_X: 64}, {T.TUNE_NT_LOAD: 1, T.TUNE_NT_STORE: 0}, {T.TUNE_VECTOR: 0},
               {T.TUNE_VECTOR: 1}, {T.TUNE_VECTOR: 2, T.TUNE_XCD: 1}, {T.TUNE_SCHEME: 1},
               {T.TUNE_SCHEME: 1, T.TUNE_NT_STORE: 0, T.TUNE_XCD: 1}]
