@@ -246,8 +246,8 @@ int plb_batch_create(int q, int variant, int ndom, const plb_domain_desc* doms,
                      const plb_boundary* bc, int device, plb_batch** out);
 int plb_batch_destroy(plb_batch* b);
 int plb_batch_initialize(plb_batch* b, double rho0, const double u0[3], const plb_init_spec* init);
-/* Tile shape: cells per tile (multiple of 64, <= 1024) and the persistent
- * grid size in blocks (0 = one block per tile). */
+/* Tile shape: cells per tile (multiple of 64 in [64, 4096]) and the
+ * persistent grid size in blocks (0 = one block per tile). */
 int plb_batch_set_tiling(plb_batch* b, int tile_cells, int grid_blocks);
 int plb_batch_get_tiling(const plb_batch* b, int* tile_cells, int* grid_blocks);
 /* Times each candidate tiling over `steps` steps and keeps the fastest.

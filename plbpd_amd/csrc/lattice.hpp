@@ -117,7 +117,8 @@ struct Prec<kF32S> {
     __device__ static S store(T t) { return t; }
 };
 // Shifted deviations g = f - w_k stored as IEEE half of g * 2^15: the scale is
-// a power of two, so it only moves g (1e-9 .. 1) into the normal half range.
+// a power of two (exact), and maps |g| from about 2e-9 up to 1 into half's
+// normal range (6.1e-5 .. 65504).
 template <>
 struct Prec<kF16S> {
     using S = _Float16;
