@@ -250,8 +250,14 @@ int plb_batch_initialize(plb_batch* b, double rho0, const double u0[3], const pl
  * persistent grid size in blocks (0 = one block per tile). */
 int plb_batch_set_tiling(plb_batch* b, int tile_cells, int grid_blocks);
 int plb_batch_get_tiling(const plb_batch* b, int* tile_cells, int* grid_blocks);
-/* Times each candidate tiling over `steps` steps and keeps the fastest.
- * results: up to max_results rows of {tile_cells, grid_blocks, ms_per_step}. */
+/* Tile order of a launch: 0 (default) = workgroup i takes tile i (persistent
+ * blocks stride by the grid), one window of memory for the chip; 1 = each XCD
+ * takes a contiguous run of tiles. */
+int plb_batch_set_mapping(plb_batch* b, int xcd_runs);
+int plb_batch_get_mapping(const plb_batch* b, int* xcd_runs);
+/* Times each candidate tiling and mapping over `steps` steps and keeps the
+ * fastest.  results: up to max_results rows of {tile_cells, grid_blocks,
+ * mapping, ms_per_step}. */
 int plb_batch_autotune(plb_batch* b, int steps, double* results, int max_results, int* nresults);
 /* nsteps of all domains: one launch per step (graph-captured when
  * steps_per_graph > 0). */

@@ -159,6 +159,8 @@ def lib():
         L.plb_batch_set_tiling.argtypes = [vp, i, i]
         L.plb_batch_get_tiling.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.plb_batch_autotune.argtypes = [vp, i, vp, i, C.POINTER(C.c_int)]
+        L.plb_batch_set_mapping.argtypes = [vp, i]
+        L.plb_batch_get_mapping.argtypes = [vp, C.POINTER(C.c_int)]
         L.plb_batch_step.argtypes = [vp, i, i]
         L.plb_batch_step_per_domain.argtypes = [vp, i]
         L.plb_batch_download_moments.argtypes = [vp, i, vp, vp]
@@ -387,8 +389,17 @@ class Batch:
         check(lib().plb_batch_get_tiling(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def autotune(self, steps=10, max_results=64):
-        res = np.zeros((max_results, 3), np.float64)
+    def set_mapping(self, xcd_runs):
+        check(lib().plb_batch_set_mapping(self.h, int(xcd_runs)))
+
+    def mapping(self):
+        m = C.c_int()
+        check(lib().plb_batch_get_mapping(self.h, C.byref(m)))
+        return m.value
+
+    def autotune(self, steps=10, max_results=128):
+        """Rows of (tile_cells, grid_blocks, mapping, ms_per_step)."""
+        res = np.zeros((max_results, 4), np.float64)
         n = C.c_int(0)
         check(lib().plb_batch_autotune(self.h, int(steps), _p(res), max_results, C.byref(n)))
         return res[: n.value]

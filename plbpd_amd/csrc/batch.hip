@@ -40,7 +40,8 @@ struct BArgs {
     const uint2* tiles;  // (domain, first cell)
     uint32_t ntiles;
     uint32_t tile;       // cells per tile
-    uint32_t persistent; // 1: gridDim.x blocks walk contiguous tile runs
+    uint32_t persistent; // 1: gridDim.x blocks walk the tiles
+    uint32_t xcd;        // 1: each XCD takes a contiguous run of tiles; 0: one window for the chip
     FusedBC<T> bc;       // shared boundary kinds
 };
 
@@ -131,17 +132,27 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch(const typename Prec<V>::S* 
                                                      typename Prec<V>::S* __restrict__ B,
                                                      BArgs<typename Prec<V>::T> a, uint32_t t_begin,
                                                      uint32_t t_end) {
-    uint32_t lo, hi;
+    // Tile order: with `xcd`, XCD i % 8 takes a contiguous run of tiles (its
+    // own window of memory, neighbouring rows in its L2); without it
+    // workgroup i takes tile i (persistent blocks stride by the grid), so the
+    // chip sweeps one contiguous window.
+    uint32_t lo, hi, stride = 1;
     const uint32_t span = t_end - t_begin;
     if (a.persistent) {
-        const uint32_t w = xcd_worker(blockIdx.x, gridDim.x);
-        lo = t_begin + (uint32_t)((uint64_t)span * w / gridDim.x);
-        hi = t_begin + (uint32_t)((uint64_t)span * (w + 1) / gridDim.x);
+        if (a.xcd) {
+            const uint32_t w = xcd_worker(blockIdx.x, gridDim.x);
+            lo = t_begin + (uint32_t)((uint64_t)span * w / gridDim.x);
+            hi = t_begin + (uint32_t)((uint64_t)span * (w + 1) / gridDim.x);
+        } else {
+            lo = t_begin + blockIdx.x;
+            hi = t_end;
+            stride = gridDim.x;
+        }
     } else {
-        lo = t_begin + xcd_worker(blockIdx.x, span);
+        lo = t_begin + (a.xcd ? xcd_worker(blockIdx.x, span) : blockIdx.x);
         hi = lo + 1;
     }
-    for (uint32_t t = lo; t < hi; ++t) {
+    for (uint32_t t = lo; t < hi; t += stride) {
         const uint2 te = a.tiles[t];
         const BDom<typename Prec<V>::T> D = a.doms[te.x];
         for (uint32_t i = threadIdx.x; i < a.tile; i += blockDim.x) {
@@ -247,6 +258,7 @@ using namespace plb;
 
 struct plb_batch {
     int q, variant, device, ndom;
+    int map;  // tile order of a launch: 0 one window, 1 XCD-contiguous runs
     std::vector<plb_domain_desc> desc;
     std::vector<uint64_t> off, ps;
     std::vector<uint32_t> tile_first;  // first tile of each domain (+ total at the end)
@@ -338,6 +350,7 @@ int batch_launch(plb_batch* b, uint32_t t0, uint32_t t1, bool persistent_ok) {
         const uint32_t span = t1 - t0;
         const bool pers = persistent_ok && b->grid > 0 && (uint32_t)b->grid < span;
         a.persistent = pers ? 1u : 0u;
+        a.xcd = b->map ? 1u : 0u;
         make_fused_bc<T>(b->bc, b->q, &a.bc);
         const unsigned threads = (unsigned)std::min(b->tile, kMaxBlock);
         const unsigned grid = pers ? (unsigned)b->grid : span;
@@ -409,6 +422,7 @@ int plb_batch_create(int q, int variant, int ndom, const plb_domain_desc* doms, 
     b->doms = nullptr;
     b->tiles = nullptr;
     b->grid = 0;
+    b->map = 0;
     b->graph = nullptr;
     b->graph_steps = 0;
     b->graph_parity = 0;
@@ -489,6 +503,22 @@ int plb_batch_get_tiling(const plb_batch* b, int* tile_cells, int* grid_blocks) 
     return PLB_OK;
 }
 
+int plb_batch_set_mapping(plb_batch* b, int xcd_runs) {
+    if (!b) return fail(PLB_ERR_INVALID_ARGUMENT, "null batch");
+    if (xcd_runs != 0 && xcd_runs != 1) return fail(PLB_ERR_INVALID_ARGUMENT, "mapping must be 0 or 1");
+    PLB_HIP_TRY(hipSetDevice(b->device));
+    PLB_HIP_TRY(hipStreamSynchronize(b->stream));
+    drop_batch_graph(b);
+    b->map = xcd_runs;
+    return PLB_OK;
+}
+
+int plb_batch_get_mapping(const plb_batch* b, int* xcd_runs) {
+    if (!b || !xcd_runs) return fail(PLB_ERR_INVALID_ARGUMENT, "null argument");
+    *xcd_runs = b->map;
+    return PLB_OK;
+}
+
 int plb_batch_step(plb_batch* b, int nsteps, int steps_per_graph) {
     if (!b || nsteps < 0) return fail(PLB_ERR_INVALID_ARGUMENT, "bad arguments");
     PLB_HIP_TRY(hipSetDevice(b->device));
@@ -540,9 +570,9 @@ int plb_batch_autotune(plb_batch* b, int steps, double* results, int max_results
     if (hipGetDeviceProperties(&p, b->device) == hipSuccess) cu = p.multiProcessorCount;
     const int tiles[] = {64, 128, 256, 512, 1024, 2048};
     const int grids[] = {0, 2 * cu, 4 * cu, 8 * cu, 16 * cu};
-    const int t_keep = b->tile, g_keep = b->grid;
+    const int t_keep = b->tile, g_keep = b->grid, m_keep = b->map;
     double best = 1e30;
-    int best_t = t_keep, best_g = g_keep, n = 0;
+    int best_t = t_keep, best_g = g_keep, best_m = m_keep, n = 0;
     hipEvent_t e0, e1;
     PLB_HIP_TRY(hipEventCreate(&e0));
     PLB_HIP_TRY(hipEventCreate(&e1));
@@ -551,8 +581,10 @@ int plb_batch_autotune(plb_batch* b, int steps, double* results, int max_results
     const int even = steps + (steps & 1);
     for (int t : tiles) {
         for (int g : grids) {
+          for (int m : {0, 1}) {
             rc = plb_batch_set_tiling(b, t, g);
             if (rc) break;
+            b->map = m;
             rc = batch_step_plain(b, 2);
             if (rc) break;
             hipError_t e = hipEventRecord(e0, b->stream);
@@ -565,16 +597,20 @@ int plb_batch_autotune(plb_batch* b, int steps, double* results, int max_results
             if (rc) break;
             const double per = (double)ms / even;
             if (results && n < max_results) {
-                results[3 * n] = t;
-                results[3 * n + 1] = g;
-                results[3 * n + 2] = per;
+                results[4 * n] = t;
+                results[4 * n + 1] = g;
+                results[4 * n + 2] = m;
+                results[4 * n + 3] = per;
             }
             ++n;
             if (per < best) {
                 best = per;
                 best_t = t;
                 best_g = g;
+                best_m = m;
             }
+          }
+          if (rc) break;
         }
         if (rc) break;
     }
@@ -582,6 +618,7 @@ int plb_batch_autotune(plb_batch* b, int steps, double* results, int max_results
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     if (nresults) *nresults = std::min(n, max_results);
+    b->map = best_m;
     return plb_batch_set_tiling(b, best_t, best_g);
 }
 

@@ -136,7 +136,8 @@ def report_c3(quick=False):
         wall, ev, scheme = bench.time_variant(P, v, mask, 200, 10, True)
         kms = ev / 200
         perf[v] = {"mlups": 256 ** 3 * 200 / wall / 1e6, "kernel_ms": kms, "scheme": scheme,
-                   "achieved_gbps": bench.algorithmic_bytes(v, nfluid) / (kms * 1e-3) / 1e9}
+                   "achieved_gbps": bench.algorithmic_bytes(v) / (kms * 1e-3) / 1e9,
+                   "fluid_only_gbps": bench.fluid_bytes(v, nfluid) / (kms * 1e-3) / 1e9}
     out["performance"] = perf
     out["speedup_f32s_vs_f64"] = perf["f32s"]["mlups"] / perf["f64"]["mlups"]
     return out
@@ -212,8 +213,10 @@ def report_c5(quick=False):
     out["total_cells"], out["fluid_cells"] = total, fluid
     b.initialize(1.0, (0, 0, 0), init)
     res = b.autotune(steps=6)
-    out["autotune"] = [{"tile_cells": int(r[0]), "grid_blocks": int(r[1]), "ms_per_step": float(r[2])} for r in res]
+    out["autotune"] = [{"tile_cells": int(r[0]), "grid_blocks": int(r[1]), "xcd_runs": int(r[2]),
+                        "ms_per_step": float(r[3])} for r in res]
     out["tiling"] = dict(zip(("tile_cells", "grid_blocks"), b.tiling()))
+    out["tiling"]["xcd_runs"] = b.mapping()
     b.initialize(1.0, (0, 0, 0), init)  # autotuning advanced the state
     b.synchronize()
     t0 = time.perf_counter()

@@ -33,17 +33,19 @@ def oracle_moments(orc, variant, dm, d, ob, steps, seed, amp):
 
 
 @pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
-@pytest.mark.parametrize("mode", ["plain", "graph", "per_domain", "persistent"])
+@pytest.mark.parametrize("mode", ["plain", "graph", "per_domain", "persistent", "plain_xcd", "persistent_xcd"])
 def test_batch_matches_oracle(orc, variant, mode):
     print(f"\n[case] batch {variant} {mode}", flush=True)
     doms = domains(7, 11)
     ob, pb = channel()
     steps, seed, amp = 12, 40, 2e-3
     b = P.Batch(19, variant, doms, pb)
-    if mode == "persistent":
+    if mode.startswith("persistent"):
         b.set_tiling(128, 3)
     elif mode == "per_domain":
         b.set_tiling(64, 0)
+    if mode.endswith("_xcd"):
+        b.set_mapping(1)
     b.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, seed, amp))
     if mode == "graph":
         b.step(steps, steps_per_graph=4)
@@ -79,10 +81,10 @@ def test_batch_autotune_and_cells():
     b = P.Batch(19, "f32s", doms, pb)
     b.initialize()
     res = b.autotune(steps=4)
-    assert len(res) > 0 and np.all(res[:, 2] > 0)
+    assert len(res) > 0 and np.all(res[:, 3] > 0)
     tile, grid = b.tiling()
-    best = res[np.argmin(res[:, 2])]
-    assert (tile, grid) == (int(best[0]), int(best[1]))
+    best = res[np.argmin(res[:, 3])]
+    assert (tile, grid, b.mapping()) == (int(best[0]), int(best[1]), int(best[2]))
     total, fluid = b.cells()
     assert total == sum(d["x"] * d["y"] * d["z"] for d in doms)
     assert fluid == sum(d["x"] * d["y"] * (d["z"] - 2) for d in doms)
