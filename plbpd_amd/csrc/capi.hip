@@ -207,9 +207,10 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->timers_on = false;
     s->tune_tx = 0;
     // defaults measured fastest on MI355X for D3Q19 256^3 (FP64, FP32, FP16
-    // storage, with and without interior solids): plain loads, non-temporal
-    // stores, linear row order, the row pull kernel and workgroup i on tile i
-    // where there are no interior solids, XCD y-stripes around solids
+    // storage, with and without interior solids): non-temporal stores, linear
+    // row order, the row pull kernel where the row length allows it (width by
+    // storage size and solids), workgroup i on tile i without interior
+    // solids and XCD y-stripes around them
     s->tune_nt_load = 0;
     s->tune_nt_store = 1;
     s->tune_scheme = PLB_SCHEME_AUTO;

@@ -594,12 +594,18 @@ enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly };
 int launch_kind(plb_state* s, KernelKind kind) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
     // the row pull kernel (fused_row.hip) takes rows of 64 * vec * (1, 2 or 4)
-    // cells.  Automatic choice, from measurement on MI355X (D3Q19 256^3): it
-    // wins without interior solids (4 cells per lane for 2-byte storage, else
-    // 1); around solids its bounce-back needs a second aligned load per
-    // direction and the cell-per-thread kernel is faster.
+    // cells.  Automatic choice, from measurement on MI355X (D3Q19 256^3):
+    // without interior solids 4 cells per lane for 2-byte storage, else 1;
+    // around solids 4 cells per lane for 4-byte storage, 2 for 8-byte storage
+    // and the cell-per-thread kernel for 2-byte storage (its bounce-back loads
+    // touch fewer lines than a second aligned row load per direction).
     int vec = kind == kPull ? s->tune_vector : 0;
-    if (vec < 0) vec = s->has_solids ? 0 : (s->sbytes == 2 ? 4 : 1);
+    if (vec < 0) {
+        if (s->has_solids)
+            vec = s->sbytes == 4 ? 4 : (s->sbytes == 8 ? 2 : 0);
+        else
+            vec = s->sbytes == 2 ? 4 : 1;
+    }
     if (vec && s->pstride % (size_t)vec) vec = 0;  // vec-wide accesses must stay aligned
     const int row_rows = vec ? pull_row_rows(vec, s->X) : 0;
     if (!row_rows) vec = 0;

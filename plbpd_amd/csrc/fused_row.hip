@@ -8,7 +8,7 @@
 // lane l of segment s owns cells x0 .. x0 + VEC - 1, x0 = (64 s + l) VEC.
 // Each population row (y - c_y, z - c_z) is loaded unshifted, so every cache
 // line is read by exactly one wavefront (which is what lets these loads be
-// non-temporal).  The x shift is done in registers: inside a lane by moving
+// non-temporal where there are no interior solids).  The x shift is done in registers: inside a lane by moving
 // values between its VEC cells, across lanes by a rotate, and across
 // segments through LDS.  The segments of a row wrap around, which is the
 // periodic x seam; a walled x axis has solid wall cells there, whose values
@@ -120,6 +120,14 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
     const bool active = row_ok && __any(any_fluid);
 
     T f[Q][VEC];
+    // Around interior solids, the bounce-back values (the cell's own reverse
+    // slots, a row that another wavefront pulls from) are loaded together
+    // with the pulled rows, so the step waits on memory once.  Those lines
+    // are read twice, so loads stay ordinary there; without interior solids
+    // every line is read once and the loads are non-temporal.
+    constexpr bool NTL = NB == 0;
+    S own[Q][VEC];
+    bool own_ok[Q];
     if (active) {
         const uint32_t yo[3] = {y == 0 ? a.Y - 1 : y - 1, y, y + 1 == a.Y ? 0 : y + 1};
         const uint32_t zo[3] = {z == 0 ? a.Z - 1 : z - 1, z, z + 1 == a.Z ? 0 : z + 1};
@@ -128,9 +136,19 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
             const int cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
             const uint64_t srow = (uint64_t)zo[1 - cz] * XY + (uint64_t)yo[1 - cy] * a.X;
             S v[VEC];
-            vload<S, VEC, true>(A + (uint64_t)k * a.pstride + srow + x0, v);
+            vload<S, VEC, NTL>(A + (uint64_t)k * a.pstride + srow + x0, v);
 #pragma unroll
             for (int j = 0; j < VEC; ++j) f[k][j] = P::load(v[j]);
+        }
+        if (NB) {
+#pragma unroll
+            for (int k = 1; k < Q; ++k) {
+                bool need = false;
+#pragma unroll
+                for (int j = 0; j < VEC; ++j) need |= ((m[j] >> rev_k<Q>(k)) & 1u) != 0 && !ss[j];
+                own_ok[k] = __any(need);
+                if (own_ok[k]) vload<S, VEC, false>(A + (uint64_t)rev_k<Q>(k) * a.pstride + c0, own[k]);
+            }
         }
         if (lane == 0 || lane == 63) {
 #pragma unroll
@@ -184,7 +202,13 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
             }
             any_bb |= bb[j] && !ss[j];
         }
-        if (__any(any_bb)) {
+        if (NB) {
+            if (k > 0 && own_ok[k]) {
+#pragma unroll
+                for (int j = 0; j < VEC; ++j)
+                    if (bb[j]) g[j] = P::load(own[k][j]);
+            }
+        } else if (__any(any_bb)) {
             S ov[VEC];
             vload<S, VEC, false>(A + (uint64_t)rev_k<Q>(k) * a.pstride + c0, ov);
 #pragma unroll
