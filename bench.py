@@ -139,7 +139,9 @@ def errors_vs_f64(P, variants, mask, steps):
     return out
 
 
-def cpu_baseline(sample_steps=20, single_core_steps=4):
+def cpu_baseline(sample_steps=150, single_core_steps=8):
+    """About 10 s of work on all host cores and 7 s on one (at about 250 and
+    20 MLUPS): a bounded sample of the same lattice, reported, not targeted."""
     from oracle.pyoracle import CpuBaseline  # the checker side: restated reference CPU path
     cb = CpuBaseline()
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))

@@ -32,6 +32,13 @@
 //    decay rate, literal == corrected in a closed box, axis-permutation
 //    symmetry, mass conservation in corrected mode.
 //  * f16 conversion: against numpy.float16 (IEEE round-to-nearest-even).
+//  * storage variants (corrected step): each within its velocity L2 bound of
+//    the FP64 run; FP64 storage with FP32 arithmetic equal to FP32 storage;
+//    exact stored values of the shifted forms (float(f - w), half(2^15 (f - w)));
+//    shifted rest state stored as zero (tests/test_oracle_variants.py).
+//  * solid voxels: a solid plane in a periodic box equals the wall-face
+//    channel; mass conserved around solids; mirror symmetry of a symmetric
+//    mask.
 //  Parity unpinned: the Q5 race of _apply_noslip (Boundary.cpp:199-220) has no
 //  single reference answer; the serial loop order restated here is one valid
 //  schedule of it.
