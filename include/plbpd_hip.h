@@ -219,10 +219,10 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                               /* linear)                                          */
 #define PLB_TUNE_VECTOR 5     /* row pull kernel with 1, 2 or 4 aligned cells per */
                               /* lane (x = 64, 128 or 256); 0 = cell-per-thread   */
-                              /* pull kernel; -1 (default) = without interior     */
-                              /* solids 4 cells per lane for 2-byte storage, else */
-                              /* 1; around solids 4 for 4-byte, 2 for 8-byte and  */
-                              /* the cell kernel for 2-byte storage               */
+                              /* pull kernel; -1 (default) = for 4- and 8-byte    */
+                              /* storage 1 without interior solids, 4 (4-byte) or */
+                              /* 2 (8-byte) around them; the cell kernel for      */
+                              /* 2-byte storage                                   */
 #define PLB_TUNE_XCD 6        /* tile mapping: 0 workgroup i takes tile i; 1 each */
                               /* XCD sweeps its own contiguous run of tiles; 2    */
                               /* each XCD takes a y-stripe of every z-plane (rows */

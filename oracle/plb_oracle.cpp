@@ -614,8 +614,10 @@ void cor_init(const Cor& g, double rho0, const double u0[3], const orc_init* in,
             }
 }
 
-// Local moments in arithmetic T (expression order of CalcMoments.hpp:27-35);
-// shifted variants sum deviations and add 1 afterwards.
+// Local moments in arithmetic T (the order of CalcMoments.hpp:27-35, k
+// ascending); shifted variants sum deviations and add 1 afterwards.  The
+// corrected step sums only the nonzero components of c_k: a product with a
+// zero component is an exact +-0 that changes at most the sign of a zero sum.
 template <class V>
 void cor_local_moments(const Lattice& L, const typename V::T* f, typename V::T* rho,
                        typename V::T* drho, typename V::T u[3]) {
@@ -624,7 +626,8 @@ void cor_local_moments(const Lattice& L, const typename V::T* f, typename V::T* 
     T j[3] = {(T)0, (T)0, (T)0};
     for (int k = 0; k < L.q; ++k) {
         r += f[k];
-        for (int i = 0; i < 3; ++i) j[i] += f[k] * (T)L.c[k][i];
+        for (int i = 0; i < 3; ++i)
+            if (L.c[k][i] != 0) j[i] += f[k] * (T)L.c[k][i];
     }
     *drho = r;
     if (V::shifted) r = (T)1 + r;
@@ -669,9 +672,15 @@ void cor_step(const Cor& g, const CorParams& p, const uint8_t* flags, const type
                 for (int i = 0; i < 3; ++i) uu[i] = u[i] + F[i] * tri;
                 const T usq = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
                 for (int k = 0; k < q; ++k) {
-                    const T cu = (k == 0) ? (T)0
-                                          : (T)L.c[k][0] * uu[0] + (T)L.c[k][1] * uu[1] +
-                                                (T)L.c[k][2] * uu[2];
+                    // c_k . u over the nonzero components, x, y, z order
+                    T cu = (T)0;
+                    bool any = false;
+                    for (int i = 0; i < 3; ++i)
+                        if (L.c[k][i] != 0) {
+                            const T t = (T)L.c[k][i] * uu[i];
+                            cu = any ? cu + t : t;
+                            any = true;
+                        }
                     const T w = (T)L.w[k];
                     T fs;
                     if (V::shifted) {

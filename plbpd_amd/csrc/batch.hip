@@ -17,6 +17,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "bgk.hpp"
 #include "state.hpp"
 
 struct plb_batch;
@@ -70,34 +71,7 @@ __device__ inline void batch_cell(const typename Prec<V>::S* __restrict__ A, typ
     T f[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) f[k] = P::load(a[(uint64_t)k * D.ps + c]);
-    T rr = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        rr += f[k];
-        j0 += f[k] * (T)Lat<Q>::cx(k);
-        j1 += f[k] * (T)Lat<Q>::cy(k);
-        j2 += f[k] * (T)Lat<Q>::cz(k);
-    }
-    const T drho = rr;
-    if (P::shifted) rr = (T)1 + rr;
-    const T ri = (T)1 / rr;
-    const T tri = D.tau / rr;
-    const T u0 = j0 * ri + D.F0 * tri;
-    const T u1 = j1 * ri + D.F1 * tri;
-    const T u2 = j2 * ri + D.F2 * tri;
-    const T usq = u0 * u0 + u1 * u1 + u2 * u2;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const T cu = (k == 0) ? (T)0 : (T)Lat<Q>::cx(k) * u0 + (T)Lat<Q>::cy(k) * u1 + (T)Lat<Q>::cz(k) * u2;
-        const T w = (T)Lat<Q>::w(k);
-        if (P::shifted) {
-            const T geq = w * (drho + rr * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
-            f[k] = D.omm * f[k] + D.om * geq;
-        } else {
-            const T neq = (w * rr) * ((((T)1 + (T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq);
-            f[k] = D.omm * f[k] + D.om * neq;
-        }
-    }
+    const T rr = bgk_update<Q, P>(f, D.tau, D.om, D.omm, D.F0, D.F1, D.F2, true);
     const uint32_t xo[3] = {x == 0 ? D.X - 1 : x - 1, x, x + 1 == D.X ? 0 : x + 1};
     const uint32_t yo[3] = {y == 0 ? D.Y - 1 : y - 1, y, y + 1 == D.Y ? 0 : y + 1};
     const uint32_t zo[3] = {z == 0 ? D.Z - 1 : z - 1, z, z + 1 == D.Z ? 0 : z + 1};
@@ -214,15 +188,11 @@ __global__ void k_batch_moments(const typename Prec<V>::S* __restrict__ f, BDom<
         for (int i = 0; i < 3; ++i) u[3 * (uint64_t)c + i] = bc.vel[1 + face][i];
         return;
     }
-    T r = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
+    T fl[Q];
 #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const T fk = P::load(f[D.off + (uint64_t)k * D.ps + c]);
-        r += fk;
-        j0 += fk * (T)Lat<Q>::cx(k);
-        j1 += fk * (T)Lat<Q>::cy(k);
-        j2 += fk * (T)Lat<Q>::cz(k);
-    }
+    for (int k = 0; k < Q; ++k) fl[k] = P::load(f[D.off + (uint64_t)k * D.ps + c]);
+    T r, j0, j1, j2;
+    moment_sums<Q, T>(fl, r, j0, j1, j2);
     if (P::shifted) r = (T)1 + r;
     const T ri = (T)1 / r;
     rho[c] = (double)r;

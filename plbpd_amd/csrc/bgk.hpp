@@ -1,0 +1,87 @@
+// The BGK update of the fused and batched kernels: moments, the
+// velocity-shift force (CollisionSRT.cpp:72-91) and the blend with the
+// equilibrium (CollisionSRT.cpp:170-175, EqlbDist.hpp:26-38), in the storage
+// variant's arithmetic T.
+//
+// Sums over directions skip the zero components of c_k (a product with a
+// zero component would be an exact +-0 whose only effect is on the sign of a
+// zero sum), so a D3Q19 moment or c.u costs its nonzero terms only; the
+// nonzero terms are taken in ascending k (moments) and x, y, z order (c.u),
+// the order of the reference's expressions.  The CPU oracle's corrected step
+// sums in exactly this way.  The literal kernels keep the reference's dense
+// expressions.
+#pragma once
+
+#include "lattice.hpp"
+
+namespace plb {
+
+// rho (or its deviation, shifted storage) and momentum j = sum_k f_k c_k.
+template <int Q, class T>
+__device__ inline void moment_sums(const T (&f)[Q], T& r, T& j0, T& j1, T& j2) {
+    r = (T)0;
+    j0 = (T)0;
+    j1 = (T)0;
+    j2 = (T)0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        r += f[k];
+        if (Lat<Q>::cx(k) != 0) j0 += f[k] * (T)Lat<Q>::cx(k);
+        if (Lat<Q>::cy(k) != 0) j1 += f[k] * (T)Lat<Q>::cy(k);
+        if (Lat<Q>::cz(k) != 0) j2 += f[k] * (T)Lat<Q>::cz(k);
+    }
+}
+
+// c_k . u over the nonzero components of c_k, in x, y, z order (0 for k = 0).
+template <int Q, class T>
+__device__ inline T c_dot(int k, T u0, T u1, T u2) {
+    const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
+    T cu = (T)0;
+    bool any = false;
+    if (cx != 0) {
+        cu = (T)cx * u0;
+        any = true;
+    }
+    if (cy != 0) {
+        cu = any ? cu + (T)cy * u1 : (T)cy * u1;
+        any = true;
+    }
+    if (cz != 0) cu = any ? cu + (T)cz * u2 : (T)cz * u2;
+    return cu;
+}
+
+// Moments and, when `collide`, the BGK update in place.  Returns rho.
+// Shifted storage sums deviations and adds 1 afterwards; its equilibrium is
+// w (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)).
+template <int Q, class P>
+__device__ inline typename P::T bgk_update(typename P::T (&f)[Q], typename P::T tau, typename P::T om,
+                                           typename P::T omm, typename P::T F0, typename P::T F1,
+                                           typename P::T F2, bool collide) {
+    using T = typename P::T;
+    T r, j0, j1, j2;
+    moment_sums<Q, T>(f, r, j0, j1, j2);
+    const T drho = r;
+    if (P::shifted) r = (T)1 + r;
+    if (!collide) return r;
+    const T ri = (T)1 / r;
+    const T tri = tau / r;
+    const T u0 = j0 * ri + F0 * tri;
+    const T u1 = j1 * ri + F1 * tri;
+    const T u2 = j2 * ri + F2 * tri;
+    const T usq = u0 * u0 + u1 * u1 + u2 * u2;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const T cu = c_dot<Q, T>(k, u0, u1, u2);
+        const T w = (T)Lat<Q>::w(k);
+        if (P::shifted) {
+            const T geq = w * (drho + r * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
+            f[k] = omm * f[k] + om * geq;
+        } else {
+            const T neq = (w * r) * ((((T)1 + (T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq);
+            f[k] = omm * f[k] + om * neq;
+        }
+    }
+    return r;
+}
+
+}  // namespace plb

@@ -180,6 +180,25 @@ __device__ inline bool pull_wall(const Seams<I>& s) {
     return w;
 }
 
+// Element i of a population buffer.  With 32-bit indices the launcher
+// guarantees that every byte offset fits 32 bits, and the access is written
+// as buffer base + 32-bit byte offset, which the compiler turns into a load
+// or store with a scalar base address (no 64-bit address add per access).
+template <class S, class I>
+__device__ inline const S* elem(const S* base, I i) {
+    if constexpr (sizeof(I) == 4)
+        return reinterpret_cast<const S*>(reinterpret_cast<const char*>(base) + i * (I)sizeof(S));
+    else
+        return base + i;
+}
+template <class S, class I>
+__device__ inline S* elem(S* base, I i) {
+    if constexpr (sizeof(I) == 4)
+        return reinterpret_cast<S*>(reinterpret_cast<char*>(base) + i * (I)sizeof(S));
+    else
+        return base + i;
+}
+
 template <int Q, int K, int V, int NB, bool NT, class I>
 __device__ inline void pull_gather(typename Prec<V>::T (&f)[Q], const typename Prec<V>::S* __restrict__ A,
                                    I c, I ps, uint32_t m, const Seams<I>& s) {
@@ -192,7 +211,8 @@ __device__ inline void pull_gather(typename Prec<V>::T (&f)[Q], const typename P
         const I own = (I)R * ps + c;
         // branch-free select (a conditional here becomes divergent branches)
         const I src = moved ^ ((moved ^ own) & ((I)0 - (I)solid));
-        f[K] = NT ? P::load(__builtin_nontemporal_load(A + src)) : P::load(A[src]);
+        const typename P::S* p = elem(A, src);
+        f[K] = NT ? P::load(__builtin_nontemporal_load(p)) : P::load(*p);
         pull_gather<Q, K + 1, V, NB, NT>(f, A, c, ps, m, s);
     }
 }
@@ -205,10 +225,11 @@ __device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Pre
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const typename P::S v = P::store(zero ? (T)0 : f[k]);
+        typename P::S* p = elem(B, (I)k * ps + c);
         if (NT)
-            __builtin_nontemporal_store(v, B + ((I)k * ps + c));
+            __builtin_nontemporal_store(v, p);
         else
-            B[(I)k * ps + c] = v;
+            *p = v;
     }
 }
 
@@ -314,15 +335,11 @@ __global__ void k_fmoments(const typename Prec<V>::S* __restrict__ f, const uint
         for (int i = 0; i < 3; ++i) u[i * n + c] = bc.vel[fl][i];
         return;
     }
-    T r = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
+    T fk[Q];
 #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const T fk = P::load(f[(uint64_t)k * pstride + c]);
-        r += fk;
-        j0 += fk * (T)Lat<Q>::cx(k);
-        j1 += fk * (T)Lat<Q>::cy(k);
-        j2 += fk * (T)Lat<Q>::cz(k);
-    }
+    for (int k = 0; k < Q; ++k) fk[k] = P::load(f[(uint64_t)k * pstride + c]);
+    T r, j0, j1, j2;
+    moment_sums<Q, T>(fk, r, j0, j1, j2);
     if (P::shifted) r = (T)1 + r;
     const T ri = (T)1 / r;
     rho[c] = (double)r;
@@ -594,17 +611,19 @@ enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly };
 int launch_kind(plb_state* s, KernelKind kind) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
     // the row pull kernel (fused_row.hip) takes rows of 64 * vec * (1, 2 or 4)
-    // cells.  Automatic choice, from measurement on MI355X (D3Q19 256^3):
-    // without interior solids 4 cells per lane for 2-byte storage, else 1;
-    // around solids 4 cells per lane for 4-byte storage, 2 for 8-byte storage
-    // and the cell-per-thread kernel for 2-byte storage (its bounce-back loads
-    // touch fewer lines than a second aligned row load per direction).
+    // cells.  Automatic choice, from measurement on MI355X (D3Q19 256^3): for
+    // 4- and 8-byte storage 1 cell per lane without interior solids, and
+    // around solids 4 (4-byte) or 2 (8-byte); for 2-byte storage, whose step
+    // is bound by instruction issue rather than HBM, the cell-per-thread
+    // kernel, which needs fewer instructions per cell.
     int vec = kind == kPull ? s->tune_vector : 0;
     if (vec < 0) {
-        if (s->has_solids)
-            vec = s->sbytes == 4 ? 4 : (s->sbytes == 8 ? 2 : 0);
+        if (s->sbytes == 2)
+            vec = 0;
+        else if (s->has_solids)
+            vec = s->sbytes == 4 ? 4 : 2;
         else
-            vec = s->sbytes == 2 ? 4 : 1;
+            vec = 1;
     }
     if (vec && s->pstride % (size_t)vec) vec = 0;  // vec-wide accesses must stay aligned
     const int row_rows = vec ? pull_row_rows(vec, s->X) : 0;
@@ -666,7 +685,8 @@ int launch_kind(plb_state* s, KernelKind kind) {
         const dim3 grid((unsigned)ntiles), block(kBlock);
         const bool nb = s->has_solids;
         // 32-bit element indices when every population index fits
-        const bool idx32 = (uint64_t)Q * s->pstride <= 0xffffffffull;
+        // 32-bit indices (and byte offsets) when every byte offset fits
+        const bool idx32 = (uint64_t)Q * s->pstride * sizeof(S) <= 0xffffffffull;
         switch (kind) {
             case kPush:
                 if (nb) hipLaunchKernelGGL((k_push<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);

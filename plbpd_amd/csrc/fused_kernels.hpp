@@ -4,6 +4,7 @@
 //   f[k * pstride + c],  c = (z*Y + y)*X + x,  x fastest.
 #pragma once
 
+#include "bgk.hpp"
 #include "state.hpp"
 
 namespace plb {
@@ -87,43 +88,11 @@ __device__ inline Cell cell_of(const FusedArgs<T>& a) {
     return p;
 }
 
-// Moments (CalcMoments.hpp:27-35 order) and, when `collide`, the BGK update
-// with the velocity-shift force (CollisionSRT.cpp:72-91, 170-175) in place.
-// Returns rho.  Shifted storage sums deviations and adds 1 afterwards.
+// Moments and, when `collide`, the BGK update with the velocity-shift force
+// in place (bgk.hpp).  Returns rho.
 template <int Q, class P>
 __device__ inline typename P::T bgk(typename P::T (&f)[Q], const FusedArgs<typename P::T>& a, bool collide) {
-    using T = typename P::T;
-    T r = (T)0, j0 = (T)0, j1 = (T)0, j2 = (T)0;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        r += f[k];
-        j0 += f[k] * (T)Lat<Q>::cx(k);
-        j1 += f[k] * (T)Lat<Q>::cy(k);
-        j2 += f[k] * (T)Lat<Q>::cz(k);
-    }
-    const T drho = r;
-    if (P::shifted) r = (T)1 + r;
-    if (!collide) return r;
-    const T ri = (T)1 / r;
-    const T tri = a.tau / r;
-    const T u0 = j0 * ri + a.F0 * tri;
-    const T u1 = j1 * ri + a.F1 * tri;
-    const T u2 = j2 * ri + a.F2 * tri;
-    const T usq = u0 * u0 + u1 * u1 + u2 * u2;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const T cu = (k == 0) ? (T)0
-                              : (T)Lat<Q>::cx(k) * u0 + (T)Lat<Q>::cy(k) * u1 + (T)Lat<Q>::cz(k) * u2;
-        const T w = (T)Lat<Q>::w(k);
-        if (P::shifted) {
-            const T geq = w * (drho + r * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
-            f[k] = a.omm * f[k] + a.om * geq;
-        } else {
-            const T neq = (w * r) * ((((T)1 + (T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq);
-            f[k] = a.omm * f[k] + a.om * neq;
-        }
-    }
-    return r;
+    return bgk_update<Q, P>(f, a.tau, a.om, a.omm, a.F0, a.F1, a.F2, collide);
 }
 
 }  // namespace
