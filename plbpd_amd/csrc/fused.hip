@@ -661,6 +661,10 @@ int launch_kind(plb_state* s, KernelKind kind) {
         a.order = 0;
         if (s->tune_order > 0 && nty % (uint32_t)s->tune_order == 0 && s->Z % s->tune_order == 0)
             a.order = (uint32_t)s->tune_order;
+        a.order_shift = a.order ? (uint32_t)__builtin_ctz(a.order) : 0u;
+        a.dntx = make_fastdiv(ntx);
+        a.dnty = make_fastdiv(nty);
+        a.dnby = make_fastdiv(a.order ? nty / a.order : 1u);
         // Automatic tile mapping: around interior solids the bounce-back loads
         // read lines again that the y +- 1 and z +- 1 rows read; XCD y-stripes
         // keep those rows in one L2 (HBM reads 1.08x the algorithmic volume
@@ -668,6 +672,7 @@ int launch_kind(plb_state* s, KernelKind kind) {
         int xm = s->tune_xcd >= 0 ? s->tune_xcd : (s->has_solids ? 2 : 0);
         if (xm == 2 && nty % 8 != 0) xm = 0;
         a.xcd = (uint32_t)xm;
+        a.dper = make_fastdiv(xm == 2 ? ntx * (nty / 8) : 1u);
         a.nt_load = s->tune_nt_load;
         a.nt_store = s->tune_nt_store;
         a.tau = (T)s->tau;

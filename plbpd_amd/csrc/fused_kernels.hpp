@@ -9,6 +9,31 @@
 
 namespace plb {
 
+// Division by a divisor fixed for a launch, as a multiply-high and two shifts
+// (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, round-up variant): exact for every 32-bit
+// dividend and divisors up to 2^31.  On block-uniform values it stays on the
+// scalar unit instead of the vector unit's emulated division.
+struct FastDiv {
+    uint32_t d, m, s1, s2;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    f.m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    f.s1 = l < 1 ? l : 1;
+    f.s2 = l > 1 ? l - 1 : 0;
+    return f;
+}
+
+__device__ inline uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    const uint32_t t = __umulhi(n, f.m);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
 template <class T>
 struct FusedArgs {
     uint32_t X, Y, Z;
@@ -16,6 +41,8 @@ struct FusedArgs {
     uint32_t tx_shift, tx_mask;  // TX = 1 << tx_shift
     uint32_t ntx, nty, ntiles;
     uint32_t order;             // 0: linear tile rows; B: B x B blocks of (y, z) rows
+    uint32_t order_shift;       // log2(order)
+    FastDiv dntx, dnty, dper, dnby;  // by ntx, nty, ntx * nty / 8, nty / order
     uint32_t xcd;               // tile mapping: 0 block i -> tile i; 1 XCD runs; 2 XCD y-stripes
     int32_t nt_load, nt_store;  // non-temporal loads / stores
     T tau, om, omm, F0, F1, F2;
@@ -45,12 +72,27 @@ __device__ inline uint32_t tile_of_block(const FusedArgs<T>& a) {
     }
     if (a.xcd == 2) {
         const uint32_t j = i & 7u, m = i >> 3;
-        const uint32_t sy = a.nty >> 3, per = a.ntx * sy;
-        const uint32_t z = m / per, r = m % per;
-        const uint32_t ty = j * sy + r / a.ntx, tx = r % a.ntx;
-        return (z * a.nty + ty) * a.ntx + tx;
+        const uint32_t sy = a.nty >> 3;
+        const uint32_t z = fdiv(m, a.dper), r = m - z * a.dper.d;
+        const uint32_t ry = fdiv(r, a.dntx), tx = r - ry * a.ntx;
+        return (z * a.nty + j * sy + ry) * a.ntx + tx;
     }
     return i;
+}
+
+// Tile row number -> (ty, tz): linear, or in square blocks of `order` x
+// `order` rows of the y-z plane.
+template <class T>
+__device__ inline void tile_rows(const FusedArgs<T>& a, uint32_t rest, uint32_t& ty, uint32_t& tz) {
+    if (a.order) {
+        const uint32_t b = rest >> (2 * a.order_shift), within = rest & (a.order * a.order - 1);
+        const uint32_t bz = fdiv(b, a.dnby), by = b - bz * a.dnby.d;
+        ty = by * a.order + (within & (a.order - 1));
+        tz = bz * a.order + (within >> a.order_shift);
+    } else {
+        tz = fdiv(rest, a.dnty);
+        ty = rest - tz * a.nty;
+    }
 }
 
 struct Cell {
@@ -66,19 +108,10 @@ struct Cell {
 template <class T>
 __device__ inline Cell cell_of(const FusedArgs<T>& a) {
     const uint32_t t = tile_of_block(a);
-    const uint32_t tx = t % a.ntx;
-    const uint32_t rest = t / a.ntx;
+    const uint32_t rest = fdiv(t, a.dntx);
+    const uint32_t tx = t - rest * a.ntx;
     uint32_t ty, tz;
-    if (a.order) {
-        const uint32_t bsz = a.order * a.order;
-        const uint32_t b = rest / bsz, within = rest % bsz;
-        const uint32_t nby = a.nty / a.order;
-        ty = (b % nby) * a.order + within % a.order;
-        tz = (b / nby) * a.order + within / a.order;
-    } else {
-        ty = rest % a.nty;
-        tz = rest / a.nty;
-    }
+    tile_rows(a, rest, ty, tz);
     Cell p;
     p.z = tz;
     p.x = (tx << a.tx_shift) + (threadIdx.x & a.tx_mask);

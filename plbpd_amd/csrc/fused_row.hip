@@ -80,16 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
 
     const uint32_t t = tile_of_block(a);
     uint32_t ty, tz;
-    if (a.order) {
-        const uint32_t bsz = a.order * a.order;
-        const uint32_t b = t / bsz, within = t % bsz;
-        const uint32_t nby = a.nty / a.order;
-        ty = (b % nby) * a.order + within % a.order;
-        tz = (b / nby) * a.order + within / a.order;
-    } else {
-        ty = t % a.nty;
-        tz = t / a.nty;
-    }
+    tile_rows(a, t, ty, tz);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t seg = wave % WPR, rib = wave / WPR;
     const uint32_t y = ty * kRows + rib, z = tz;
