@@ -38,13 +38,6 @@ int plb_set_tuning(plb_state* s, int key, int value) {
         case PLB_TUNE_SCHEME:
             if (value != PLB_SCHEME_AUTO && value != PLB_SCHEME_PUSH && value != PLB_SCHEME_PULL)
                 return fail(PLB_ERR_INVALID_ARGUMENT, "unknown streaming scheme");
-            if (value == PLB_SCHEME_PULL) {
-                for (int f = 0; f < 6; ++f)
-                    if (s->bc_set && s->bc.type[f] == kNoslip)
-                        for (int d = 0; d < 3; ++d)
-                            if (s->bc.vel[f][d] != 0.0)
-                                return fail(PLB_ERR_LOGIC, "the pull step has no moving-wall term");
-            }
             s->tune_scheme = value;
             break;
         case PLB_TUNE_TILE_ORDER:

@@ -233,7 +233,50 @@ __device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Pre
     }
 }
 
-// Pull step on post-collision storage (stationary walls): each fluid cell
+// Bit k set when the neighbour x + c_k is not fluid (the neighbour mask
+// k_nbmask stores), from the flags.
+template <int Q, class I>
+__device__ inline uint32_t nb_mask_of(const uint8_t* __restrict__ flags, I c, const Seams<I>& s);
+template <int Q, int K, class I>
+__device__ inline uint32_t nb_bits(const uint8_t* __restrict__ flags, I c, const Seams<I>& s) {
+    if constexpr (K < Q) {
+        const uint32_t b = flags[c + pull_offset<Q, rev_k<Q>(K)>(s)] ? (1u << K) : 0u;
+        return b | nb_bits<Q, K + 1>(flags, c, s);
+    } else {
+        return 0u;
+    }
+}
+template <int Q, class I>
+__device__ inline uint32_t nb_mask_of(const uint8_t* __restrict__ flags, I c, const Seams<I>& s) {
+    return nb_bits<Q, 1>(flags, c, s);
+}
+
+// Moving walls in the pull scheme.  A population f*_k that a fluid cell sends
+// towards a wall or solid neighbour comes back to the same cell as direction
+// rev(k) at the next step (halfway bounce-back), read from the cell's own
+// slot k, and nothing else reads that slot.  So the moving-wall term of
+// Boundary.cpp:213-217 is subtracted when the slot is written, with the rho
+// and the expression of the push step (which subtracts it on the way): the
+// two schemes store the same values.
+template <int Q, int K, int NB, class T, class I>
+__device__ inline void wall_terms(T (&f)[Q], T rho, const uint8_t* __restrict__ flags, I c, uint32_t m,
+                                  const Seams<I>& s, const FusedBC<T>& bc) {
+    if constexpr (K < Q) {
+        constexpr int R = rev_k<Q>(K);
+        constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
+        // neighbour x + c_K, the pull source of direction rev(K)
+        const bool wall = NB ? ((m >> K) & 1u) != 0 : pull_wall<Q, R>(s);
+        if (wall) {
+            const int fl = flags[c + pull_offset<Q, R>(s)];
+            const T cub = (T)cx * bc.vel[fl][0] + (T)cy * bc.vel[fl][1] + (T)cz * bc.vel[fl][2];
+            const T term = (T)2 * (T)Lat<Q>::w(K) * rho * cub * (T)3;
+            f[K] = f[K] - term;
+        }
+        wall_terms<Q, K + 1, NB>(f, rho, flags, c, m, s, bc);
+    }
+}
+
+// Pull step on post-collision storage: each fluid cell
 // gathers population k from x - c_k, or from its own reverse slot when that
 // cell is solid (the same halfway bounce-back as the push step), then
 // collides and stores every population in its own slot.  A wavefront writes
@@ -270,7 +313,10 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
     const I ps = (I)a.pstride;
     T f[Q];
     pull_gather<Q, 0, V, NB, (NTM & 1) != 0>(f, A, c, ps, m, sm);
-    if (COLLIDE) bgk<Q, P>(f, a, true);
+    if (COLLIDE) {
+        const T rho = bgk<Q, P>(f, a, true);
+        if (a.bc.moving && !self_solid) wall_terms<Q, 1, NB>(f, rho, flags, c, m, sm, a.bc);
+    }
     own_store<Q, V, (NTM & 2) != 0>(f, B, c, ps, self_solid);
 }
 
@@ -286,7 +332,14 @@ __global__ __launch_bounds__(kBlock) void k_collide_only(typename Prec<V>::S* __
     T f[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) f[k] = P::load(A[(uint64_t)k * a.pstride + p.c]);
-    bgk<Q, P>(f, a, true);
+    const T rho = bgk<Q, P>(f, a, true);
+    if (a.bc.moving) {
+        // the first pull-scheme step: the same write-time moving-wall terms as
+        // k_pull, with every non-fluid neighbour taken from the flags
+        const Seams<uint64_t> sm = seams<uint64_t>(p.x, p.y, p.z, a);
+        const uint64_t m = nb_mask_of<Q>(flags, p.c, sm);
+        wall_terms<Q, 1, 1>(f, rho, flags, p.c, (uint32_t)m, sm, a.bc);
+    }
 #pragma unroll
     for (int k = 0; k < Q; ++k) A[(uint64_t)k * a.pstride + p.c] = P::store(f[k]);
 }
@@ -753,21 +806,13 @@ int launch_kind(plb_state* s, KernelKind kind) {
     return rc;
 }
 
-bool moving_walls(const plb_state* s) {
-    for (int f = 0; f < 6; ++f)
-        if (s->bc.type[f] == kNoslip)
-            for (int d = 0; d < 3; ++d)
-                if (s->bc.vel[f][d] != 0.0) return true;
-    return false;
-}
 
 }  // namespace
 
 bool fused_uses_pull(const plb_state* s) {
-    if (s->tune_scheme == PLB_SCHEME_PUSH) return false;
-    // the pull step has no moving-wall term; otherwise it is the faster step
-    // for every storage type on MI355X (whole-line stores around solids)
-    return !moving_walls(s);
+    // the pull step is the faster one for every storage type and boundary on
+    // MI355X (whole-line stores; moving walls by write-time terms)
+    return s->tune_scheme != PLB_SCHEME_PUSH;
 }
 
 // The populations in buf[cur] are stored either before collision (what the

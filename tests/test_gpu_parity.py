@@ -184,8 +184,7 @@ def fused_bc(q, walls, lid=(0.03, 0.0, 0.0)):
 @pytest.mark.parametrize("scheme", ["push", "pull"])
 def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
     print(f"\n[case] fused {scheme} {variant} q={q} {X}x{Y}x{Z} walls={walls} solids={solids}", flush=True)
-    lid = (0.03, 0.0, 0.0) if scheme == "push" else (0.0, 0.0, 0.0)  # pull: stationary walls
-    ob, pb = fused_bc(q, walls, lid)
+    ob, pb = fused_bc(q, walls, (0.03, 0.0, 0.0))  # moving lid (both schemes)
     tau, F = 0.62, (2e-5, 1e-6 if q != 9 else 0.0, -1e-6)
     mask = None
     if solids:
@@ -218,7 +217,7 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
 @pytest.mark.parametrize("X,vec", [(64, 1), (128, 1), (128, 2), (256, 1), (256, 2), (256, 4)])
 @pytest.mark.parametrize("nt", [0, 1])
 def test_closed_box_vector_pull(orc, variant, X, vec, nt):
-    """All six faces stationary walls: bounce-back on x-walls happens per
+    """All six faces walls (the lid moving): bounce-back on x-walls happens per
     element inside the row pull kernel, whose x shifts cross lanes and (for
     rows of several wavefronts) segments; with and without non-temporal
     loads and stores."""
@@ -226,6 +225,7 @@ def test_closed_box_vector_pull(orc, variant, X, vec, nt):
     q, Y, Z = 19, 6, 5
     types = [NOSLIP] * 6
     vel = {f: (0.0, 0.0, 0.0) for f in range(6)}
+    vel[UP] = (0.04, -0.01, 0.0)  # a moving lid: write-time wall terms in the row kernel
     ob, pb = both_bc(types, vel)
     rng = np.random.default_rng(2)
     mask = (rng.random((Z, Y, X)) < 0.05).astype(np.uint8)
@@ -249,12 +249,13 @@ def test_closed_box_vector_pull(orc, variant, X, vec, nt):
 
 @pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
 @pytest.mark.parametrize("solids", [False, True])
-def test_tuning_knobs_keep_results(orc, variant, solids):
+@pytest.mark.parametrize("lid", [0.0, 0.03])
+def test_tuning_knobs_keep_results(orc, variant, solids, lid):
     """Every launch-shape knob (XCD mapping, tile order, block width,
     non-temporal loads and stores, row or cell kernel, push or pull) gives the
     oracle's populations bit for bit."""
     q, X, Y, Z = 19, 128, 16, 8  # 8 tile rows: the XCD y-stripe mapping applies
-    ob, pb = fused_bc(q, True, lid=(0.0, 0.0, 0.0))
+    ob, pb = fused_bc(q, True, lid=(lid, 0.0, 0.0))
     mask = None
     if solids:
         mask = (np.random.default_rng(4).random((Z, Y, X)) < 0.1).astype(np.uint8)
@@ -269,7 +270,7 @@ def test_tuning_knobs_keep_results(orc, variant, solids):
               {T.TUNE_VECTOR: 1}, {T.TUNE_VECTOR: 2, T.TUNE_XCD: 1}, {T.TUNE_SCHEME: 1},
               {T.TUNE_SCHEME: 1, T.TUNE_NT_STORE: 0, T.TUNE_XCD: 1}]
     for knobs in combos:
-        print(f"\n[case] knobs {variant} solids={solids} {knobs}", flush=True)
+        print(f"\n[case] knobs {variant} solids={solids} lid={lid} {knobs}", flush=True)
         with P.State(q, X, Y, Z, variant, P.FUSED) as st:
             for k, v in knobs.items():
                 st.set_tuning(k, v)
