@@ -640,6 +640,7 @@ struct CorParams {
     double tau;
     double F[3];
     double vel[8][3];  // by flag value
+    bool moving;       // some wall face has a nonzero velocity
 };
 
 template <class V>
@@ -697,12 +698,19 @@ void cor_step(const Cor& g, const CorParams& p, const uint8_t* flags, const type
                     const size_t d = g.cell(zd, yd, xd);
                     const int fl = flags[d];
                     if (fl) {
-                        const double* ubd = p.vel[fl];
-                        const T ub[3] = {(T)ubd[0], (T)ubd[1], (T)ubd[2]};
-                        const T cub = (T)L.c[k][0] * ub[0] + (T)L.c[k][1] * ub[1] +
-                                      (T)L.c[k][2] * ub[2];
-                        const T term = (T)2 * w * rho * cub * (T)3;
-                        B[q * c + L.rev[k]] = V::store(fs - term);
+                        // halfway bounce-back; the moving-wall term of
+                        // Boundary.cpp:213-217 where a wall face (flags 1-6,
+                        // not an interior solid, 7) moves
+                        T v = fs;
+                        if (p.moving && fl <= 6) {
+                            const double* ubd = p.vel[fl];
+                            const T ub[3] = {(T)ubd[0], (T)ubd[1], (T)ubd[2]};
+                            const T cub = (T)L.c[k][0] * ub[0] + (T)L.c[k][1] * ub[1] +
+                                          (T)L.c[k][2] * ub[2];
+                            const T term = (T)2 * w * rho * cub * (T)3;
+                            v = fs - term;
+                        }
+                        B[q * c + L.rev[k]] = V::store(v);
                     } else {
                         B[q * d + k] = V::store(fs);
                     }
@@ -758,8 +766,12 @@ void cor_params(const orc_bc* bc, double tau, const double F[3], CorParams* p) {
     std::memset(p, 0, sizeof(*p));
     p->tau = tau;
     for (int i = 0; i < 3; ++i) p->F[i] = F[i];
+    p->moving = false;
     for (int f = 0; f < 6; ++f)
-        for (int i = 0; i < 3; ++i) p->vel[1 + f][i] = bc->has_vel[f] ? bc->vel[f][i] : 0.0;
+        for (int i = 0; i < 3; ++i) {
+            p->vel[1 + f][i] = bc->has_vel[f] ? bc->vel[f][i] : 0.0;
+            if (bc->type[f] == kNoslip && p->vel[1 + f][i] != 0.0) p->moving = true;
+        }
 }
 
 template <class Fn>

@@ -103,7 +103,9 @@ __global__ __launch_bounds__(kBlock) void k_push(const typename Prec<V>::S* __re
         const uint64_t d = (uint64_t)nb.zo[cz + 1] * XY + (uint64_t)nb.yo[cy + 1] * a.X + nb.xo[cx + 1];
         const bool solid = NB ? ((m >> k) & 1u) != 0 : (nb.wx[cx + 1] || nb.wy[cy + 1] || nb.wz[cz + 1]);
         T v = f[k];
-        if (a.bc.moving && solid && !self_solid) {
+        // the moving-wall term where a wall face (not an interior solid) moves
+        const bool face_wall = nb.wx[cx + 1] || nb.wy[cy + 1] || nb.wz[cz + 1];
+        if (a.bc.moving && face_wall && !self_solid) {
             const int fl = flags[d];
             const T cub = (T)cx * a.bc.vel[fl][0] + (T)cy * a.bc.vel[fl][1] + (T)cz * a.bc.vel[fl][2];
             const T term = (T)2 * (T)Lat<Q>::w(k) * rho * cub * (T)3;
@@ -233,46 +235,27 @@ __device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Pre
     }
 }
 
-// Bit k set when the neighbour x + c_k is not fluid (the neighbour mask
-// k_nbmask stores), from the flags.
-template <int Q, class I>
-__device__ inline uint32_t nb_mask_of(const uint8_t* __restrict__ flags, I c, const Seams<I>& s);
-template <int Q, int K, class I>
-__device__ inline uint32_t nb_bits(const uint8_t* __restrict__ flags, I c, const Seams<I>& s) {
-    if constexpr (K < Q) {
-        const uint32_t b = flags[c + pull_offset<Q, rev_k<Q>(K)>(s)] ? (1u << K) : 0u;
-        return b | nb_bits<Q, K + 1>(flags, c, s);
-    } else {
-        return 0u;
-    }
-}
-template <int Q, class I>
-__device__ inline uint32_t nb_mask_of(const uint8_t* __restrict__ flags, I c, const Seams<I>& s) {
-    return nb_bits<Q, 1>(flags, c, s);
-}
-
 // Moving walls in the pull scheme.  A population f*_k that a fluid cell sends
-// towards a wall or solid neighbour comes back to the same cell as direction
-// rev(k) at the next step (halfway bounce-back), read from the cell's own
-// slot k, and nothing else reads that slot.  So the moving-wall term of
-// Boundary.cpp:213-217 is subtracted when the slot is written, with the rho
-// and the expression of the push step (which subtracts it on the way): the
-// two schemes store the same values.
-template <int Q, int K, int NB, class T, class I>
-__device__ inline void wall_terms(T (&f)[Q], T rho, const uint8_t* __restrict__ flags, I c, uint32_t m,
+// towards a wall comes back to the same cell as direction rev(k) at the next
+// step (halfway bounce-back), read from the cell's own slot k, and nothing
+// else reads that slot.  So the moving-wall term of Boundary.cpp:213-217 is
+// subtracted when the slot is written, with the rho and the expression of the
+// push step (which subtracts it on the way): the two schemes store the same
+// values.  Only wall faces move; interior solids take no term.
+template <int Q, int K, class T, class I>
+__device__ inline void wall_terms(T (&f)[Q], T rho, const uint8_t* __restrict__ flags, I c,
                                   const Seams<I>& s, const FusedBC<T>& bc) {
     if constexpr (K < Q) {
         constexpr int R = rev_k<Q>(K);
         constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
-        // neighbour x + c_K, the pull source of direction rev(K)
-        const bool wall = NB ? ((m >> K) & 1u) != 0 : pull_wall<Q, R>(s);
-        if (wall) {
+        // neighbour x + c_K (the pull source of direction rev(K)) on a wall face
+        if (pull_wall<Q, R>(s)) {
             const int fl = flags[c + pull_offset<Q, R>(s)];
             const T cub = (T)cx * bc.vel[fl][0] + (T)cy * bc.vel[fl][1] + (T)cz * bc.vel[fl][2];
             const T term = (T)2 * (T)Lat<Q>::w(K) * rho * cub * (T)3;
             f[K] = f[K] - term;
         }
-        wall_terms<Q, K + 1, NB>(f, rho, flags, c, m, s, bc);
+        wall_terms<Q, K + 1>(f, rho, flags, c, s, bc);
     }
 }
 
@@ -315,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
     pull_gather<Q, 0, V, NB, (NTM & 1) != 0>(f, A, c, ps, m, sm);
     if (COLLIDE) {
         const T rho = bgk<Q, P>(f, a, true);
-        if (a.bc.moving && !self_solid) wall_terms<Q, 1, NB>(f, rho, flags, c, m, sm, a.bc);
+        if (a.bc.moving && !self_solid) wall_terms<Q, 1>(f, rho, flags, c, sm, a.bc);
     }
     own_store<Q, V, (NTM & 2) != 0>(f, B, c, ps, self_solid);
 }
@@ -334,11 +317,9 @@ __global__ __launch_bounds__(kBlock) void k_collide_only(typename Prec<V>::S* __
     for (int k = 0; k < Q; ++k) f[k] = P::load(A[(uint64_t)k * a.pstride + p.c]);
     const T rho = bgk<Q, P>(f, a, true);
     if (a.bc.moving) {
-        // the first pull-scheme step: the same write-time moving-wall terms as
-        // k_pull, with every non-fluid neighbour taken from the flags
+        // the first pull-scheme step: the same write-time moving-wall terms as k_pull
         const Seams<uint64_t> sm = seams<uint64_t>(p.x, p.y, p.z, a);
-        const uint64_t m = nb_mask_of<Q>(flags, p.c, sm);
-        wall_terms<Q, 1, 1>(f, rho, flags, p.c, (uint32_t)m, sm, a.bc);
+        wall_terms<Q, 1>(f, rho, flags, p.c, sm, a.bc);
     }
 #pragma unroll
     for (int k = 0; k < Q; ++k) A[(uint64_t)k * a.pstride + p.c] = P::store(f[k]);

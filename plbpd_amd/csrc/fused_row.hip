@@ -65,22 +65,19 @@ __device__ inline void vstore(S* p, const S (&v)[VEC]) {
 // Moving-wall terms subtracted when the slots that point into a wall are
 // written (the pull-scheme bounce-back reads them back from the same cell;
 // the same values as the push step, see wall_terms in fused.hip).
-template <int Q, int NB, class T>
-__device__ inline void row_wall_terms(T (&f)[Q], T rho, const uint8_t* __restrict__ flags, uint32_t m, uint32_t x,
-                                      uint32_t y, uint32_t z, const FusedArgs<T>& a) {
+template <int Q, class T>
+__device__ inline void row_wall_terms(T (&f)[Q], T rho, const uint8_t* __restrict__ flags, uint32_t x, uint32_t y,
+                                      uint32_t z, const FusedArgs<T>& a) {
     const uint32_t xo[3] = {x == 0 ? a.X - 1 : x - 1, x, x + 1 == a.X ? 0 : x + 1};
     const uint32_t yo[3] = {y == 0 ? a.Y - 1 : y - 1, y, y + 1 == a.Y ? 0 : y + 1};
     const uint32_t zo[3] = {z == 0 ? a.Z - 1 : z - 1, z, z + 1 == a.Z ? 0 : z + 1};
 #pragma unroll
     for (int k = 1; k < Q; ++k) {
         const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
-        bool wall;
-        if (NB)
-            wall = ((m >> k) & 1u) != 0;
-        else
-            wall = (cx > 0 && a.bc.wall_hi[0] && x + 2 == a.X) || (cx < 0 && a.bc.wall_lo[0] && x == 1) ||
-                   (cy > 0 && a.bc.wall_hi[1] && y + 2 == a.Y) || (cy < 0 && a.bc.wall_lo[1] && y == 1) ||
-                   (cz > 0 && a.bc.wall_hi[2] && z + 2 == a.Z) || (cz < 0 && a.bc.wall_lo[2] && z == 1);
+        // the neighbour x + c_k on a wall face (interior solids take no term)
+        const bool wall = (cx > 0 && a.bc.wall_hi[0] && x + 2 == a.X) || (cx < 0 && a.bc.wall_lo[0] && x == 1) ||
+                          (cy > 0 && a.bc.wall_hi[1] && y + 2 == a.Y) || (cy < 0 && a.bc.wall_lo[1] && y == 1) ||
+                          (cz > 0 && a.bc.wall_hi[2] && z + 2 == a.Z) || (cz < 0 && a.bc.wall_lo[2] && z == 1);
         if (wall) {
             const uint64_t d = ((uint64_t)zo[1 + cz] * a.Y + yo[1 + cy]) * a.X + xo[1 + cx];
             const int fl = flags[d];
@@ -245,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
 #pragma unroll
         for (int k = 0; k < Q; ++k) fj[k] = f[k][j];
         const T rho = bgk<Q, P>(fj, a, true);
-        if (a.bc.moving && !ss[j]) row_wall_terms<Q, NB>(fj, rho, flags, m[j], x0 + j, y, z, a);
+        if (a.bc.moving && !ss[j]) row_wall_terms<Q>(fj, rho, flags, x0 + j, y, z, a);
 #pragma unroll
         for (int k = 0; k < Q; ++k) f[k][j] = fj[k];
     }

@@ -233,11 +233,11 @@ void test_literal_march_matches_oracle() {
     EXPECT(std::memcmp(n.data(), sim.n->data(), 19 * cells * sizeof(double)) == 0);
 }
 
-void test_fused_march_matches_oracle(int variant) {
+void test_fused_march_matches_oracle(int variant, float lid) {
     const int X = 16, Y = 8, Z = 10, steps = 30;
     const double tau = 0.65;
     const std::array<double, 3> F = {2e-5, 0.0, 0.0};
-    Setup s = couette(X, Y, Z, tau, F, 0.0f);
+    Setup s = couette(X, Y, Z, tau, F, lid);
     plbpd::DeviceConfig cfg;
     cfg.mode = PLB_MODE_FUSED;
     cfg.variant = variant;
@@ -318,7 +318,8 @@ int main(int argc, char** argv) {
     }
     if (what == "gpu" || what == "all") {
         test_literal_march_matches_oracle();
-        for (int v : {PLB_F64, PLB_F32S, PLB_F16S}) test_fused_march_matches_oracle(v);
+        for (int v : {PLB_F64, PLB_F32S, PLB_F16S}) test_fused_march_matches_oracle(v, 0.0f);
+        for (int v : {PLB_F64, PLB_F32S}) test_fused_march_matches_oracle(v, 0.03f);  // moving lid
         test_device_errors();
         test_state_file_resume();
     }
