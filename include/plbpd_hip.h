@@ -211,7 +211,7 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                             /* kernel's aligned loads are always non-temporal.  */
 #define PLB_TUNE_NT_STORE 2 /* non-temporal population stores (0/1; default 1) */
 #define PLB_TUNE_SCHEME 3   /* streaming scheme of the fused step:              */
-#define PLB_SCHEME_AUTO 0   /*   pull unless a wall moves (default)             */
+#define PLB_SCHEME_AUTO 0   /*   pull (default; moving walls included)         */
 #define PLB_SCHEME_PUSH 1   /*   collide, then scatter to neighbours            */
 #define PLB_SCHEME_PULL 2   /*   gather from neighbours, then collide           */
 #define PLB_TUNE_TILE_ORDER 4 /* rows in B x B blocks of the y-z plane (B a power */
