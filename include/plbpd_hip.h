@@ -206,9 +206,11 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
 /* Fused-kernel tuning knobs.  The defaults are the settings measured fastest
  * on MI355X for D3Q19 256^3; a change drops a captured step graph. */
 #define PLB_TUNE_BLOCK_X 0  /* cells per block row: 64, 128, 256 (0 = from x)   */
-#define PLB_TUNE_NT_LOAD 1  /* non-temporal population loads of the cell-per-    */
-                            /* thread kernels (0/1; default 0).  The row pull    */
-                            /* kernel's aligned loads are always non-temporal.  */
+#define PLB_TUNE_NT_LOAD 1  /* non-temporal population loads of the push kernel */
+                            /* (0/1; default 0).  The cell pull kernel's loads   */
+                            /* are ordinary (its shifted lines are shared with   */
+                            /* the next wavefront); the row pull kernel's are    */
+                            /* non-temporal without interior solids.            */
 #define PLB_TUNE_NT_STORE 2 /* non-temporal population stores (0/1; default 1) */
 #define PLB_TUNE_SCHEME 3   /* streaming scheme of the fused step:              */
 #define PLB_SCHEME_AUTO 0   /*   pull (default; moving walls included)         */

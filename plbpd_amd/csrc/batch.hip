@@ -70,7 +70,7 @@ __device__ inline void batch_cell(const typename Prec<V>::S* __restrict__ A, typ
     typename P::S* b = B + D.off;
     T f[Q];
 #pragma unroll
-    for (int k = 0; k < Q; ++k) f[k] = P::load(a[(uint64_t)k * D.ps + c]);
+    for (int k = 0; k < Q; ++k) f[k] = P::load_raw(a[(uint64_t)k * D.ps + c]);
     const T rr = bgk_update<Q, P>(f, D.tau, D.om, D.omm, D.F0, D.F1, D.F2, true);
     const uint32_t xo[3] = {x == 0 ? D.X - 1 : x - 1, x, x + 1 == D.X ? 0 : x + 1};
     const uint32_t yo[3] = {y == 0 ? D.Y - 1 : y - 1, y, y + 1 == D.Y ? 0 : y + 1};
@@ -93,11 +93,11 @@ __device__ inline void batch_cell(const typename Prec<V>::S* __restrict__ A, typ
                 if (on[fc]) face = fc;
             const T* ub = bc.vel[1 + face];
             const T cub = (T)cx * ub[0] + (T)cy * ub[1] + (T)cz * ub[2];
-            v = v - (T)2 * (T)Lat<Q>::w(k) * rr * cub * (T)3;
+            v = v - (T)2 * w_scaled<Q, P>(k) * rr * cub * (T)3;
         }
         const uint32_t d = (zo[cz + 1] * D.Y + yo[cy + 1]) * D.X + xo[cx + 1];
         const uint64_t dst = solid ? (uint64_t)rev<Q>(k) * D.ps + c : (uint64_t)k * D.ps + d;
-        b[dst] = P::store(v);
+        b[dst] = P::store_raw(v);
     }
 }
 

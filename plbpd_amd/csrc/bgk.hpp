@@ -50,9 +50,18 @@ __device__ inline T c_dot(int k, T u0, T u1, T u2) {
     return cu;
 }
 
-// Moments and, when `collide`, the BGK update in place.  Returns rho.
+// Weight w_k times the storage scale (exact: a power of two).
+template <int Q, class P>
+__device__ inline typename P::T w_scaled(int k) {
+    return (typename P::T)Lat<Q>::w(k) * P::kScale;
+}
+
+// Moments and, when `collide`, the BGK update in place, on f in the storage
+// scale (P::load_raw: kScale * f, kScale a power of two).  Returns rho.
 // Shifted storage sums deviations and adds 1 afterwards; its equilibrium is
-// w (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)).
+// w (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)).  Every scaled quantity is
+// kScale times its unscaled value exactly, so the stored results equal those
+// of the same arithmetic on unscaled values.
 template <int Q, class P>
 __device__ inline typename P::T bgk_update(typename P::T (&f)[Q], typename P::T tau, typename P::T om,
                                            typename P::T omm, typename P::T F0, typename P::T F1,
@@ -60,6 +69,13 @@ __device__ inline typename P::T bgk_update(typename P::T (&f)[Q], typename P::T 
     using T = typename P::T;
     T r, j0, j1, j2;
     moment_sums<Q, T>(f, r, j0, j1, j2);
+    if (P::kScale != (T)1) {
+        const T inv = (T)1 / P::kScale;
+        r *= inv;
+        j0 *= inv;
+        j1 *= inv;
+        j2 *= inv;
+    }
     const T drho = r;
     if (P::shifted) r = (T)1 + r;
     if (!collide) return r;
@@ -72,7 +88,7 @@ __device__ inline typename P::T bgk_update(typename P::T (&f)[Q], typename P::T 
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const T cu = c_dot<Q, T>(k, u0, u1, u2);
-        const T w = (T)Lat<Q>::w(k);
+        const T w = w_scaled<Q, P>(k);
         if (P::shifted) {
             const T geq = w * (drho + r * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
             f[k] = omm * f[k] + om * geq;

@@ -87,10 +87,10 @@ __global__ __launch_bounds__(kBlock) void k_push(const typename Prec<V>::S* __re
         for (int k = 0; k < Q; ++k) f[k] = (T)0;
     } else if (a.nt_load) {
 #pragma unroll
-        for (int k = 0; k < Q; ++k) f[k] = P::load(__builtin_nontemporal_load(A + (uint64_t)k * a.pstride + c));
+        for (int k = 0; k < Q; ++k) f[k] = P::load_raw(__builtin_nontemporal_load(A + (uint64_t)k * a.pstride + c));
     } else {
 #pragma unroll
-        for (int k = 0; k < Q; ++k) f[k] = P::load(A[(uint64_t)k * a.pstride + c]);
+        for (int k = 0; k < Q; ++k) f[k] = P::load_raw(A[(uint64_t)k * a.pstride + c]);
     }
     T rho = (T)0;
     if (COLLIDE || a.bc.moving) rho = bgk<Q, P>(f, a, COLLIDE);
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_push(const typename Prec<V>::S* __re
         if (a.bc.moving && face_wall && !self_solid) {
             const int fl = flags[d];
             const T cub = (T)cx * a.bc.vel[fl][0] + (T)cy * a.bc.vel[fl][1] + (T)cz * a.bc.vel[fl][2];
-            const T term = (T)2 * (T)Lat<Q>::w(k) * rho * cub * (T)3;
+            const T term = (T)2 * w_scaled<Q, P>(k) * rho * cub * (T)3;
             v = v - term;
         }
         // one store per direction, whatever the lane's case
@@ -118,9 +118,9 @@ __global__ __launch_bounds__(kBlock) void k_push(const typename Prec<V>::S* __re
             v = (T)0;
         }
         if (a.nt_store)
-            __builtin_nontemporal_store(P::store(v), B + dst);
+            __builtin_nontemporal_store(P::store_raw(v), B + dst);
         else
-            B[dst] = P::store(v);
+            B[dst] = P::store_raw(v);
     }
 }
 
@@ -214,7 +214,7 @@ __device__ inline void pull_gather(typename Prec<V>::T (&f)[Q], const typename P
         // branch-free select (a conditional here becomes divergent branches)
         const I src = moved ^ ((moved ^ own) & ((I)0 - (I)solid));
         const typename P::S* p = elem(A, src);
-        f[K] = NT ? P::load(__builtin_nontemporal_load(p)) : P::load(*p);
+        f[K] = NT ? P::load_raw(__builtin_nontemporal_load(p)) : P::load_raw(*p);
         pull_gather<Q, K + 1, V, NB, NT>(f, A, c, ps, m, s);
     }
 }
@@ -226,7 +226,7 @@ __device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Pre
     using T = typename P::T;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        const typename P::S v = P::store(zero ? (T)0 : f[k]);
+        const typename P::S v = P::store_raw(zero ? (T)0 : f[k]);
         typename P::S* p = elem(B, (I)k * ps + c);
         if (NT)
             __builtin_nontemporal_store(v, p);
@@ -242,9 +242,10 @@ __device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Pre
 // subtracted when the slot is written, with the rho and the expression of the
 // push step (which subtracts it on the way): the two schemes store the same
 // values.  Only wall faces move; interior solids take no term.
-template <int Q, int K, class T, class I>
-__device__ inline void wall_terms(T (&f)[Q], T rho, const uint8_t* __restrict__ flags, I c,
-                                  const Seams<I>& s, const FusedBC<T>& bc) {
+template <int Q, int K, class P, class I>
+__device__ inline void wall_terms(typename P::T (&f)[Q], typename P::T rho, const uint8_t* __restrict__ flags, I c,
+                                  const Seams<I>& s, const FusedBC<typename P::T>& bc) {
+    using T = typename P::T;
     if constexpr (K < Q) {
         constexpr int R = rev_k<Q>(K);
         constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
@@ -252,10 +253,10 @@ __device__ inline void wall_terms(T (&f)[Q], T rho, const uint8_t* __restrict__ 
         if (pull_wall<Q, R>(s)) {
             const int fl = flags[c + pull_offset<Q, R>(s)];
             const T cub = (T)cx * bc.vel[fl][0] + (T)cy * bc.vel[fl][1] + (T)cz * bc.vel[fl][2];
-            const T term = (T)2 * (T)Lat<Q>::w(K) * rho * cub * (T)3;
+            const T term = (T)2 * w_scaled<Q, P>(K) * rho * cub * (T)3;
             f[K] = f[K] - term;
         }
-        wall_terms<Q, K + 1>(f, rho, flags, c, s, bc);
+        wall_terms<Q, K + 1, P>(f, rho, flags, c, s, bc);
     }
 }
 
@@ -271,7 +272,7 @@ __device__ inline void wall_terms(T (&f)[Q], T rho, const uint8_t* __restrict__ 
 // offsets are per-cell constants (seams()), indices are 32-bit (I = uint32_t)
 // whenever Q * pstride fits, and loads are unconditional (every index is in
 // range; solid lanes discard what they load and store zeros).
-template <int Q, int V, int NB, bool COLLIDE, class I, int NTM>
+template <int Q, int V, int NB, bool COLLIDE, class I, bool NTS, bool MOV>
 __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __restrict__ A,
                                                  typename Prec<V>::S* __restrict__ B,
                                                  const uint8_t* __restrict__ flags,
@@ -295,12 +296,15 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
     const Seams<I> sm = seams<I>(p.x, p.y, p.z, a);
     const I ps = (I)a.pstride;
     T f[Q];
-    pull_gather<Q, 0, V, NB, (NTM & 1) != 0>(f, A, c, ps, m, sm);
+    pull_gather<Q, 0, V, NB, false>(f, A, c, ps, m, sm);
     if (COLLIDE) {
         const T rho = bgk<Q, P>(f, a, true);
-        if (a.bc.moving && !self_solid) wall_terms<Q, 1>(f, rho, flags, c, sm, a.bc);
+        // MOV: an instantiation for boundaries with a moving wall, so that
+        // the others carry no wall-term code (it costs 2-byte storage, whose
+        // step is issue-bound, about 5%)
+        if (MOV && a.bc.moving && !self_solid) wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
     }
-    own_store<Q, V, (NTM & 2) != 0>(f, B, c, ps, self_solid);
+    own_store<Q, V, NTS>(f, B, c, ps, self_solid);
 }
 
 // Collision only, in place (pre-collision -> post-collision storage).
@@ -314,15 +318,15 @@ __global__ __launch_bounds__(kBlock) void k_collide_only(typename Prec<V>::S* __
     if (!p.valid || flags[p.c]) return;
     T f[Q];
 #pragma unroll
-    for (int k = 0; k < Q; ++k) f[k] = P::load(A[(uint64_t)k * a.pstride + p.c]);
+    for (int k = 0; k < Q; ++k) f[k] = P::load_raw(A[(uint64_t)k * a.pstride + p.c]);
     const T rho = bgk<Q, P>(f, a, true);
     if (a.bc.moving) {
         // the first pull-scheme step: the same write-time moving-wall terms as k_pull
         const Seams<uint64_t> sm = seams<uint64_t>(p.x, p.y, p.z, a);
-        wall_terms<Q, 1>(f, rho, flags, p.c, sm, a.bc);
+        wall_terms<Q, 1, P>(f, rho, flags, p.c, sm, a.bc);
     }
 #pragma unroll
-    for (int k = 0; k < Q; ++k) A[(uint64_t)k * a.pstride + p.c] = P::store(f[k]);
+    for (int k = 0; k < Q; ++k) A[(uint64_t)k * a.pstride + p.c] = P::store_raw(f[k]);
 }
 
 // Initial state: populations from feq(rho0, u(cell)) evaluated in double
@@ -739,40 +743,37 @@ int launch_kind(plb_state* s, KernelKind kind) {
                 if (vec) {
                     launch_pull_row<Q, V>(a, vec, a.nt_store != 0, nb, A, B, s->flags, s->nbmask, grid.x, s->stream);
                 } else if (idx32) {
-                    // non-temporal loads (bit 0) / stores (bit 1) on the hot path only
-                    const int ntm = (a.nt_load ? 1 : 0) | (a.nt_store ? 2 : 0);
-                    auto go = [&](auto nbc, auto ntc) {
-                        constexpr int NBv = decltype(nbc)::value, NTv = decltype(ntc)::value;
-                        hipLaunchKernelGGL((k_pull<Q, V, NBv, true, uint32_t, NTv>), grid, block, 0, s->stream, A, B,
-                                           s->flags, s->nbmask, a);
+                    // non-temporal stores and moving walls by instantiation on the hot path
+                    auto go = [&](auto nbc, auto ntc, auto mvc) {
+                        constexpr int NBv = decltype(nbc)::value;
+                        constexpr bool NTv = decltype(ntc)::value, MVv = decltype(mvc)::value;
+                        hipLaunchKernelGGL((k_pull<Q, V, NBv, true, uint32_t, NTv, MVv>), grid, block, 0, s->stream,
+                                           A, B, s->flags, s->nbmask, a);
                     };
-                    using I0 = std::integral_constant<int, 0>;
-                    using I1 = std::integral_constant<int, 1>;
-                    using I2 = std::integral_constant<int, 2>;
-                    using I3 = std::integral_constant<int, 3>;
+                    using N0 = std::integral_constant<int, 0>;
+                    using N1 = std::integral_constant<int, 1>;
+                    using F = std::false_type;
+                    using Tr = std::true_type;
+                    const bool nts = a.nt_store != 0, mv = a.bc.moving != 0;
                     if (nb) {
-                        if (ntm == 0) go(I1{}, I0{});
-                        else if (ntm == 1) go(I1{}, I1{});
-                        else if (ntm == 2) go(I1{}, I2{});
-                        else go(I1{}, I3{});
+                        if (nts) mv ? go(N1{}, Tr{}, Tr{}) : go(N1{}, Tr{}, F{});
+                        else mv ? go(N1{}, F{}, Tr{}) : go(N1{}, F{}, F{});
                     } else {
-                        if (ntm == 0) go(I0{}, I0{});
-                        else if (ntm == 1) go(I0{}, I1{});
-                        else if (ntm == 2) go(I0{}, I2{});
-                        else go(I0{}, I3{});
+                        if (nts) mv ? go(N0{}, Tr{}, Tr{}) : go(N0{}, Tr{}, F{});
+                        else mv ? go(N0{}, F{}, Tr{}) : go(N0{}, F{}, F{});
                     }
                 } else {
-                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, true, uint64_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-                    else hipLaunchKernelGGL((k_pull<Q, V, 0, true, uint64_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, true, uint64_t, false, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    else hipLaunchKernelGGL((k_pull<Q, V, 0, true, uint64_t, false, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
                 }
                 break;
             case kGather:
                 if (idx32) {
-                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, false, uint32_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-                    else hipLaunchKernelGGL((k_pull<Q, V, 0, false, uint32_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, false, uint32_t, false, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    else hipLaunchKernelGGL((k_pull<Q, V, 0, false, uint32_t, false, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
                 } else {
-                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, false, uint64_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
-                    else hipLaunchKernelGGL((k_pull<Q, V, 0, false, uint64_t, 0>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    if (nb) hipLaunchKernelGGL((k_pull<Q, V, 1, false, uint64_t, false, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
+                    else hipLaunchKernelGGL((k_pull<Q, V, 0, false, uint64_t, false, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
                 }
                 break;
             case kCollideOnly:

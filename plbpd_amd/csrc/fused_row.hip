@@ -65,9 +65,10 @@ __device__ inline void vstore(S* p, const S (&v)[VEC]) {
 // Moving-wall terms subtracted when the slots that point into a wall are
 // written (the pull-scheme bounce-back reads them back from the same cell;
 // the same values as the push step, see wall_terms in fused.hip).
-template <int Q, class T>
-__device__ inline void row_wall_terms(T (&f)[Q], T rho, const uint8_t* __restrict__ flags, uint32_t x, uint32_t y,
-                                      uint32_t z, const FusedArgs<T>& a) {
+template <int Q, class P>
+__device__ inline void row_wall_terms(typename P::T (&f)[Q], typename P::T rho, const uint8_t* __restrict__ flags,
+                                      uint32_t x, uint32_t y, uint32_t z, const FusedArgs<typename P::T>& a) {
+    using T = typename P::T;
     const uint32_t xo[3] = {x == 0 ? a.X - 1 : x - 1, x, x + 1 == a.X ? 0 : x + 1};
     const uint32_t yo[3] = {y == 0 ? a.Y - 1 : y - 1, y, y + 1 == a.Y ? 0 : y + 1};
     const uint32_t zo[3] = {z == 0 ? a.Z - 1 : z - 1, z, z + 1 == a.Z ? 0 : z + 1};
@@ -82,7 +83,7 @@ __device__ inline void row_wall_terms(T (&f)[Q], T rho, const uint8_t* __restric
             const uint64_t d = ((uint64_t)zo[1 + cz] * a.Y + yo[1 + cy]) * a.X + xo[1 + cx];
             const int fl = flags[d];
             const T cub = (T)cx * a.bc.vel[fl][0] + (T)cy * a.bc.vel[fl][1] + (T)cz * a.bc.vel[fl][2];
-            const T term = (T)2 * (T)Lat<Q>::w(k) * rho * cub * (T)3;
+            const T term = (T)2 * w_scaled<Q, P>(k) * rho * cub * (T)3;
             f[k] = f[k] - term;
         }
     }
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
             S v[VEC];
             vload<S, VEC, NTL>(A + (uint64_t)k * a.pstride + srow + x0, v);
 #pragma unroll
-            for (int j = 0; j < VEC; ++j) f[k][j] = P::load(v[j]);
+            for (int j = 0; j < VEC; ++j) f[k][j] = P::load_raw(v[j]);
         }
         if (NB) {
 #pragma unroll
@@ -223,14 +224,14 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
             if (k > 0 && own_ok[k]) {
 #pragma unroll
                 for (int j = 0; j < VEC; ++j)
-                    if (bb[j]) g[j] = P::load(own[k][j]);
+                    if (bb[j]) g[j] = P::load_raw(own[k][j]);
             }
         } else if (__any(any_bb)) {
             S ov[VEC];
             vload<S, VEC, false>(A + (uint64_t)rev_k<Q>(k) * a.pstride + c0, ov);
 #pragma unroll
             for (int j = 0; j < VEC; ++j)
-                if (bb[j]) g[j] = P::load(ov[j]);
+                if (bb[j]) g[j] = P::load_raw(ov[j]);
         }
 #pragma unroll
         for (int j = 0; j < VEC; ++j) f[k][j] = g[j];
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
 #pragma unroll
         for (int k = 0; k < Q; ++k) fj[k] = f[k][j];
         const T rho = bgk<Q, P>(fj, a, true);
-        if (a.bc.moving && !ss[j]) row_wall_terms<Q>(fj, rho, flags, x0 + j, y, z, a);
+        if (a.bc.moving && !ss[j]) row_wall_terms<Q, P>(fj, rho, flags, x0 + j, y, z, a);
 #pragma unroll
         for (int k = 0; k < Q; ++k) f[k][j] = fj[k];
     }
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
     for (int k = 0; k < Q; ++k) {
         S o[VEC];
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) o[j] = P::store(ss[j] ? (T)0 : f[k][j]);
+        for (int j = 0; j < VEC; ++j) o[j] = P::store_raw(ss[j] ? (T)0 : f[k][j]);
         vstore<S, VEC, NTS>(B + (uint64_t)k * a.pstride + c0, o);
     }
 }

@@ -79,6 +79,11 @@ __host__ __device__ constexpr int rev(int k) {
 
 // ------------------------------------------------------ storage variants --
 // Variant ids match plb_variant in include/plbpd_hip.h.
+//   load/store       stored value <-> f (or the deviation g = f - w_k)
+//   load_raw/...     stored value <-> kScale * g, the form the fused and
+//                    batched steps compute in (kScale is a power of two, so
+//                    scaling is exact and commutes with rounding: the step's
+//                    results are bit for bit those of computing on g)
 enum Variant { kF64 = 0, kF32 = 1, kF64A32 = 2, kF32S = 3, kF16S = 4 };
 
 template <int V>
@@ -91,6 +96,9 @@ struct Prec<kF64> {
     static constexpr bool shifted = false;
     __device__ static T load(S s) { return s; }
     __device__ static S store(T t) { return t; }
+    static constexpr T kScale = 1.0;
+    __device__ static T load_raw(S s) { return s; }
+    __device__ static S store_raw(T t) { return t; }
 };
 template <>
 struct Prec<kF64A32> {
@@ -99,6 +107,9 @@ struct Prec<kF64A32> {
     static constexpr bool shifted = false;
     __device__ static T load(S s) { return (float)s; }
     __device__ static S store(T t) { return (double)t; }
+    static constexpr T kScale = 1.0f;
+    __device__ static T load_raw(S s) { return (float)s; }
+    __device__ static S store_raw(T t) { return (double)t; }
 };
 template <>
 struct Prec<kF32> {
@@ -107,6 +118,9 @@ struct Prec<kF32> {
     static constexpr bool shifted = false;
     __device__ static T load(S s) { return s; }
     __device__ static S store(T t) { return t; }
+    static constexpr T kScale = 1.0f;
+    __device__ static T load_raw(S s) { return s; }
+    __device__ static S store_raw(T t) { return t; }
 };
 template <>
 struct Prec<kF32S> {
@@ -115,6 +129,9 @@ struct Prec<kF32S> {
     static constexpr bool shifted = true;
     __device__ static T load(S s) { return s; }
     __device__ static S store(T t) { return t; }
+    static constexpr T kScale = 1.0f;
+    __device__ static T load_raw(S s) { return s; }
+    __device__ static S store_raw(T t) { return t; }
 };
 // Shifted deviations g = f - w_k stored as IEEE half of g * 2^15: the scale is
 // a power of two (exact), and maps |g| from about 2e-9 up to 1 into half's
@@ -126,6 +143,9 @@ struct Prec<kF16S> {
     static constexpr bool shifted = true;
     __device__ static T load(S s) { return (float)s * 0x1.0p-15f; }
     __device__ static S store(T t) { return (_Float16)(t * 0x1.0p15f); }
+    static constexpr T kScale = 0x1.0p15f;
+    __device__ static T load_raw(S s) { return (float)s; }
+    __device__ static S store_raw(T t) { return (_Float16)t; }
 };
 
 inline size_t storage_bytes(int v) {
