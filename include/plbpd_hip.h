@@ -253,6 +253,10 @@ int plb_batch_initialize(plb_batch* b, double rho0, const double u0[3], const pl
  * persistent grid size in blocks (0 = one block per tile). */
 int plb_batch_set_tiling(plb_batch* b, int tile_cells, int grid_blocks);
 int plb_batch_get_tiling(const plb_batch* b, int* tile_cells, int* grid_blocks);
+/* Streaming scheme of the batched step: PLB_SCHEME_AUTO / PLB_SCHEME_PULL
+ * (default: gather, whole-line stores, moving walls by write-time terms) or
+ * PLB_SCHEME_PUSH. */
+int plb_batch_set_scheme(plb_batch* b, int scheme);
 /* Tile order of a launch: 0 (default) = workgroup i takes tile i (persistent
  * blocks stride by the grid), one window of memory for the chip; 1 = each XCD
  * takes a contiguous run of tiles. */

@@ -160,6 +160,7 @@ def lib():
         L.plb_batch_get_tiling.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.plb_batch_autotune.argtypes = [vp, i, vp, i, C.POINTER(C.c_int)]
         L.plb_batch_set_mapping.argtypes = [vp, i]
+        L.plb_batch_set_scheme.argtypes = [vp, i]
         L.plb_batch_get_mapping.argtypes = [vp, C.POINTER(C.c_int)]
         L.plb_batch_step.argtypes = [vp, i, i]
         L.plb_batch_step_per_domain.argtypes = [vp, i]
@@ -388,6 +389,9 @@ class Batch:
         a, b = C.c_int(), C.c_int()
         check(lib().plb_batch_get_tiling(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def set_scheme(self, scheme):
+        check(lib().plb_batch_set_scheme(self.h, int(scheme)))
 
     def set_mapping(self, xcd_runs):
         check(lib().plb_batch_set_mapping(self.h, int(xcd_runs)))

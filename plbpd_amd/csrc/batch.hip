@@ -101,15 +101,129 @@ __device__ inline void batch_cell(const typename Prec<V>::S* __restrict__ A, typ
     }
 }
 
+// The face a wall cell belongs to: the last no-slip face in reference order
+// wins (Boundary.cpp:17-18), as the flags of fused.hip.
+template <class T>
+__device__ inline int wall_face(const BDom<T>& D, const FusedBC<T>& bc, uint32_t x, uint32_t y, uint32_t z) {
+    int face = -1;
+    const bool on[6] = {x == D.X - 1 && bc.wall_hi[0], x == 0 && bc.wall_lo[0], y == D.Y - 1 && bc.wall_hi[1],
+                        y == 0 && bc.wall_lo[1], z == D.Z - 1 && bc.wall_hi[2], z == 0 && bc.wall_lo[2]};
+    for (int fc = 0; fc < 6; ++fc)
+        if (on[fc]) face = fc;
+    return face;
+}
+
+// Pull step of one cell (see k_pull in fused.hip): gather population k from
+// x - c_k, or from the cell's own slot rev(k) where that neighbour is a wall
+// layer (halfway bounce-back); then, when COLLIDE, the BGK update and, for a
+// moving wall, the wall term subtracted from the slot that the cell reads
+// back at the next step; store into the cell's own slots.  Wall cells store
+// zeros, so that every line of B is written whole.
+template <int Q, int V, bool COLLIDE>
+__device__ inline void batch_pull_cell(const typename Prec<V>::S* __restrict__ A,
+                                       typename Prec<V>::S* __restrict__ B, const BDom<typename Prec<V>::T>& D,
+                                       const FusedBC<typename Prec<V>::T>& bc, uint32_t c) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    const uint32_t x = c % D.X;
+    const uint32_t r = c / D.X;
+    const uint32_t y = r % D.Y;
+    const uint32_t z = r / D.Y;
+    const typename P::S* a = A + D.off;
+    typename P::S* b = B + D.off;
+    const bool solid_self = (bc.wall_lo[0] && x == 0) || (bc.wall_hi[0] && x == D.X - 1) ||
+                            (bc.wall_lo[1] && y == 0) || (bc.wall_hi[1] && y == D.Y - 1) ||
+                            (bc.wall_lo[2] && z == 0) || (bc.wall_hi[2] && z == D.Z - 1);
+    if (solid_self) {
+#pragma unroll
+        for (int k = 0; k < Q; ++k) b[(uint64_t)k * D.ps + c] = P::store_raw((T)0);
+        return;
+    }
+    const uint32_t xo[3] = {x == 0 ? D.X - 1 : x - 1, x, x + 1 == D.X ? 0 : x + 1};
+    const uint32_t yo[3] = {y == 0 ? D.Y - 1 : y - 1, y, y + 1 == D.Y ? 0 : y + 1};
+    const uint32_t zo[3] = {z == 0 ? D.Z - 1 : z - 1, z, z + 1 == D.Z ? 0 : z + 1};
+    const bool wx[3] = {bc.wall_lo[0] && xo[0] == 0, false, bc.wall_hi[0] && xo[2] == D.X - 1};
+    const bool wy[3] = {bc.wall_lo[1] && yo[0] == 0, false, bc.wall_hi[1] && yo[2] == D.Y - 1};
+    const bool wz[3] = {bc.wall_lo[2] && zo[0] == 0, false, bc.wall_hi[2] && zo[2] == D.Z - 1};
+    T f[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
+        const bool bb = wx[1 - cx] || wy[1 - cy] || wz[1 - cz];
+        const uint32_t s = (zo[1 - cz] * D.Y + yo[1 - cy]) * D.X + xo[1 - cx];
+        const uint64_t src = bb ? (uint64_t)rev<Q>(k) * D.ps + c : (uint64_t)k * D.ps + s;
+        f[k] = P::load_raw(a[src]);
+    }
+    if (COLLIDE) {
+        const T rr = bgk_update<Q, P>(f, D.tau, D.om, D.omm, D.F0, D.F1, D.F2, true);
+        if (bc.moving) {
+#pragma unroll
+            for (int k = 1; k < Q; ++k) {
+                const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
+                if (wx[cx + 1] || wy[cy + 1] || wz[cz + 1]) {
+                    const int face = wall_face(D, bc, xo[cx + 1], yo[cy + 1], zo[cz + 1]);
+                    const T* ub = bc.vel[1 + face];
+                    const T cub = (T)cx * ub[0] + (T)cy * ub[1] + (T)cz * ub[2];
+                    f[k] = f[k] - (T)2 * w_scaled<Q, P>(k) * rr * cub * (T)3;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) b[(uint64_t)k * D.ps + c] = P::store_raw(f[k]);
+}
+
+// The first pull-scheme step from pre-collision storage: the collision alone,
+// in place, with the same write-time wall terms.
 template <int Q, int V>
-__global__ __launch_bounds__(kMaxBlock) void k_batch(const typename Prec<V>::S* __restrict__ A,
-                                                     typename Prec<V>::S* __restrict__ B,
-                                                     BArgs<typename Prec<V>::T> a, uint32_t t_begin,
-                                                     uint32_t t_end) {
-    // Tile order: with `xcd`, XCD i % 8 takes a contiguous run of tiles (its
-    // own window of memory, neighbouring rows in its L2); without it
-    // workgroup i takes tile i (persistent blocks stride by the grid), so the
-    // chip sweeps one contiguous window.
+__device__ inline void batch_collide_cell(typename Prec<V>::S* A, const BDom<typename Prec<V>::T>& D,
+                                          const FusedBC<typename Prec<V>::T>& bc, uint32_t c) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    const uint32_t x = c % D.X;
+    const uint32_t r = c / D.X;
+    const uint32_t y = r % D.Y;
+    const uint32_t z = r / D.Y;
+    const bool solid_self = (bc.wall_lo[0] && x == 0) || (bc.wall_hi[0] && x == D.X - 1) ||
+                            (bc.wall_lo[1] && y == 0) || (bc.wall_hi[1] && y == D.Y - 1) ||
+                            (bc.wall_lo[2] && z == 0) || (bc.wall_hi[2] && z == D.Z - 1);
+    if (solid_self) return;
+    typename P::S* a = A + D.off;
+    T f[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) f[k] = P::load_raw(a[(uint64_t)k * D.ps + c]);
+    const T rr = bgk_update<Q, P>(f, D.tau, D.om, D.omm, D.F0, D.F1, D.F2, true);
+    if (bc.moving) {
+        const uint32_t xo[3] = {x == 0 ? D.X - 1 : x - 1, x, x + 1 == D.X ? 0 : x + 1};
+        const uint32_t yo[3] = {y == 0 ? D.Y - 1 : y - 1, y, y + 1 == D.Y ? 0 : y + 1};
+        const uint32_t zo[3] = {z == 0 ? D.Z - 1 : z - 1, z, z + 1 == D.Z ? 0 : z + 1};
+        const bool wx[3] = {bc.wall_lo[0] && xo[0] == 0, false, bc.wall_hi[0] && xo[2] == D.X - 1};
+        const bool wy[3] = {bc.wall_lo[1] && yo[0] == 0, false, bc.wall_hi[1] && yo[2] == D.Y - 1};
+        const bool wz[3] = {bc.wall_lo[2] && zo[0] == 0, false, bc.wall_hi[2] && zo[2] == D.Z - 1};
+#pragma unroll
+        for (int k = 1; k < Q; ++k) {
+            const int cx = Lat<Q>::cx(k), cy = Lat<Q>::cy(k), cz = Lat<Q>::cz(k);
+            if (wx[cx + 1] || wy[cy + 1] || wz[cz + 1]) {
+                const int face = wall_face(D, bc, xo[cx + 1], yo[cy + 1], zo[cz + 1]);
+                const T* ub = bc.vel[1 + face];
+                const T cub = (T)cx * ub[0] + (T)cy * ub[1] + (T)cz * ub[2];
+                f[k] = f[k] - (T)2 * w_scaled<Q, P>(k) * rr * cub * (T)3;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) a[(uint64_t)k * D.ps + c] = P::store_raw(f[k]);
+}
+
+// Kinds of batched launch.
+enum BatchKind { kBPush = 0, kBPull = 1, kBGather = 2, kBCollide = 3 };
+
+// The tiles of one launch: with `xcd`, XCD i % 8 takes a contiguous run of
+// tiles (its own window of memory, neighbouring rows in its L2); without it
+// workgroup i takes tile i (persistent blocks stride by the grid), so the chip
+// sweeps one contiguous window.  fn(domain, cell) for every cell.
+template <class T, class Fn>
+__device__ inline void batch_tiles(const BArgs<T>& a, uint32_t t_begin, uint32_t t_end, Fn&& fn) {
     uint32_t lo, hi, stride = 1;
     const uint32_t span = t_end - t_begin;
     if (a.persistent) {
@@ -128,13 +242,37 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch(const typename Prec<V>::S* 
     }
     for (uint32_t t = lo; t < hi; t += stride) {
         const uint2 te = a.tiles[t];
-        const BDom<typename Prec<V>::T> D = a.doms[te.x];
+        const BDom<T> D = a.doms[te.x];
         for (uint32_t i = threadIdx.x; i < a.tile; i += blockDim.x) {
             const uint32_t c = te.y + i;
             if (c >= D.N) break;
-            batch_cell<Q, V>(A, B, D, a.bc, c);
+            fn(D, c);
         }
     }
+}
+
+// One step (push or pull) or the gather, A -> B.
+template <int Q, int V, int KIND>
+__global__ __launch_bounds__(kMaxBlock) void k_batch(const typename Prec<V>::S* __restrict__ A,
+                                                     typename Prec<V>::S* __restrict__ B,
+                                                     BArgs<typename Prec<V>::T> a, uint32_t t_begin,
+                                                     uint32_t t_end) {
+    batch_tiles(a, t_begin, t_end, [&](const BDom<typename Prec<V>::T>& D, uint32_t c) {
+        if (KIND == kBPush)
+            batch_cell<Q, V>(A, B, D, a.bc, c);
+        else if (KIND == kBPull)
+            batch_pull_cell<Q, V, true>(A, B, D, a.bc, c);
+        else
+            batch_pull_cell<Q, V, false>(A, B, D, a.bc, c);
+    });
+}
+
+// The collision alone, in place.
+template <int Q, int V>
+__global__ __launch_bounds__(kMaxBlock) void k_batch_collide(typename Prec<V>::S* A, BArgs<typename Prec<V>::T> a,
+                                                             uint32_t t_begin, uint32_t t_end) {
+    batch_tiles(a, t_begin, t_end,
+                [&](const BDom<typename Prec<V>::T>& D, uint32_t c) { batch_collide_cell<Q, V>(A, D, a.bc, c); });
 }
 
 template <int Q, int V>
@@ -228,7 +366,9 @@ using namespace plb;
 
 struct plb_batch {
     int q, variant, device, ndom;
-    int map;  // tile order of a launch: 0 one window, 1 XCD-contiguous runs
+    int map;     // tile order of a launch: 0 one window, 1 XCD-contiguous runs
+    int scheme;  // PLB_SCHEME_*: pull (default) or push
+    bool post;   // pull scheme: buffers hold post-collision populations
     std::vector<plb_domain_desc> desc;
     std::vector<uint64_t> off, ps;
     std::vector<uint32_t> tile_first;  // first tile of each domain (+ total at the end)
@@ -305,7 +445,7 @@ void drop_batch_graph(plb_batch* b) {
 }
 
 // One launch over tiles [t0, t1) (all domains, or one domain for the baseline).
-int batch_launch(plb_batch* b, uint32_t t0, uint32_t t1, bool persistent_ok) {
+int batch_launch(plb_batch* b, uint32_t t0, uint32_t t1, bool persistent_ok, int kind) {
     const int rc = with_qv_b(b->q, b->variant, [&](auto qc, auto vc) {
         constexpr int Q = decltype(qc)::value;
         constexpr int V = decltype(vc)::value;
@@ -324,19 +464,71 @@ int batch_launch(plb_batch* b, uint32_t t0, uint32_t t1, bool persistent_ok) {
         make_fused_bc<T>(b->bc, b->q, &a.bc);
         const unsigned threads = (unsigned)std::min(b->tile, kMaxBlock);
         const unsigned grid = pers ? (unsigned)b->grid : span;
-        hipLaunchKernelGGL((k_batch<Q, V>), dim3(grid), dim3(threads), 0, b->stream, (const S*)b->buf[b->cur],
-                           (S*)b->buf[b->cur ^ 1], a, t0, t1);
+        const S* A = (const S*)b->buf[b->cur];
+        S* B = (S*)b->buf[b->cur ^ 1];
+        switch (kind) {
+            case kBPush:
+                hipLaunchKernelGGL((k_batch<Q, V, kBPush>), dim3(grid), dim3(threads), 0, b->stream, A, B, a, t0, t1);
+                break;
+            case kBPull:
+                hipLaunchKernelGGL((k_batch<Q, V, kBPull>), dim3(grid), dim3(threads), 0, b->stream, A, B, a, t0, t1);
+                break;
+            case kBGather:
+                hipLaunchKernelGGL((k_batch<Q, V, kBGather>), dim3(grid), dim3(threads), 0, b->stream, A, B, a, t0,
+                                   t1);
+                break;
+            default:  // in place on the current buffer
+                hipLaunchKernelGGL((k_batch_collide<Q, V>), dim3(grid), dim3(threads), 0, b->stream,
+                                   (S*)b->buf[b->cur], a, t0, t1);
+                break;
+        }
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
     return rc;
 }
 
+bool batch_pull(const plb_batch* b) { return b->scheme != PLB_SCHEME_PUSH; }
+
+// Pull-scheme storage between steps is post-collision with the streaming of
+// the step pending (see fused_to_pre in fused.hip); the gather finishes it.
+int batch_to_pre(plb_batch* b) {
+    if (!b->post) return PLB_OK;
+    const int rc = batch_launch(b, 0, b->ntiles, true, kBGather);
+    if (rc) return rc;
+    b->cur ^= 1;
+    b->post = false;
+    return PLB_OK;
+}
+
+// One step over tiles [t0, t1) of every domain range given, in the batch's
+// scheme.  A pull-scheme step from pre-collision storage is the collision
+// alone (in place); later pull steps gather, then collide.
+template <class Launch>
+int batch_one_step(plb_batch* b, Launch&& launch) {
+    int rc;
+    if (batch_pull(b)) {
+        if (!b->post) {
+            rc = launch(kBCollide);
+            if (rc) return rc;
+            b->post = true;
+            return PLB_OK;
+        }
+        rc = launch(kBPull);
+    } else {
+        rc = batch_to_pre(b);
+        if (rc) return rc;
+        rc = launch(kBPush);
+    }
+    if (rc) return rc;
+    b->cur ^= 1;
+    return PLB_OK;
+}
+
 int batch_step_plain(plb_batch* b, int nsteps) {
     for (int i = 0; i < nsteps; ++i) {
-        const int rc = batch_launch(b, 0, b->ntiles, true);
+        const int rc = batch_one_step(b, [&](int kind) { return batch_launch(b, 0, b->ntiles, true, kind); });
         if (rc) return rc;
-        b->cur ^= 1;
     }
     return PLB_OK;
 }
@@ -393,6 +585,8 @@ int plb_batch_create(int q, int variant, int ndom, const plb_domain_desc* doms, 
     b->tiles = nullptr;
     b->grid = 0;
     b->map = 0;
+    b->scheme = PLB_SCHEME_AUTO;
+    b->post = false;
     b->graph = nullptr;
     b->graph_steps = 0;
     b->graph_parity = 0;
@@ -449,7 +643,18 @@ int plb_batch_initialize(plb_batch* b, double rho0, const double u0[3], const pl
         return PLB_OK;
     });
     if (rc) return rc;
+    b->post = false;
     PLB_HIP_TRY(hipStreamSynchronize(b->stream));
+    return PLB_OK;
+}
+
+int plb_batch_set_scheme(plb_batch* b, int scheme) {
+    if (!b) return fail(PLB_ERR_INVALID_ARGUMENT, "null batch");
+    if (scheme != PLB_SCHEME_AUTO && scheme != PLB_SCHEME_PUSH && scheme != PLB_SCHEME_PULL)
+        return fail(PLB_ERR_INVALID_ARGUMENT, "unknown streaming scheme");
+    PLB_HIP_TRY(hipSetDevice(b->device));
+    drop_batch_graph(b);
+    b->scheme = scheme;
     return PLB_OK;
 }
 
@@ -494,6 +699,19 @@ int plb_batch_step(plb_batch* b, int nsteps, int steps_per_graph) {
     PLB_HIP_TRY(hipSetDevice(b->device));
     if (steps_per_graph <= 0) return batch_step_plain(b, nsteps);
     if (steps_per_graph & 1) return fail(PLB_ERR_INVALID_ARGUMENT, "steps_per_graph must be even");
+    // the captured steps keep one storage form: post-collision for the pull
+    // step (from pre-collision storage its first step, the collision alone,
+    // runs here), pre-collision for the push step
+    if (batch_pull(b)) {
+        if (!b->post && nsteps > 0) {
+            const int rc = batch_step_plain(b, 1);
+            if (rc) return rc;
+            --nsteps;
+        }
+    } else {
+        const int rc = batch_to_pre(b);
+        if (rc) return rc;
+    }
     const int reps = nsteps / steps_per_graph;
     if (reps > 0 && (!b->graph || b->graph_steps != steps_per_graph || b->graph_parity != b->cur)) {
         drop_batch_graph(b);
@@ -523,11 +741,14 @@ int plb_batch_step_per_domain(plb_batch* b, int nsteps) {
     if (!b || nsteps < 0) return fail(PLB_ERR_INVALID_ARGUMENT, "bad arguments");
     PLB_HIP_TRY(hipSetDevice(b->device));
     for (int i = 0; i < nsteps; ++i) {
-        for (int d = 0; d < b->ndom; ++d) {
-            const int rc = batch_launch(b, b->tile_first[d], b->tile_first[d + 1], false);
-            if (rc) return rc;
-        }
-        b->cur ^= 1;
+        const int rc = batch_one_step(b, [&](int kind) {
+            for (int d = 0; d < b->ndom; ++d) {
+                const int lrc = batch_launch(b, b->tile_first[d], b->tile_first[d + 1], false, kind);
+                if (lrc) return lrc;
+            }
+            return PLB_OK;
+        });
+        if (rc) return rc;
     }
     return PLB_OK;
 }
@@ -595,6 +816,8 @@ int plb_batch_autotune(plb_batch* b, int steps, double* results, int max_results
 int plb_batch_download_moments(plb_batch* b, int d, double* rho, double* u) {
     if (!b || !rho || !u || d < 0 || d >= b->ndom) return fail(PLB_ERR_INVALID_ARGUMENT, "bad arguments");
     PLB_HIP_TRY(hipSetDevice(b->device));
+    const int prc = batch_to_pre(b);
+    if (prc) return prc;
     const uint64_t n = (uint64_t)b->desc[d].x * b->desc[d].y * b->desc[d].z;
     double* dev = nullptr;
     PLB_HIP_TRY(hipMalloc(&dev, n * 4 * sizeof(double)));

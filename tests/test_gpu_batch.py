@@ -34,12 +34,14 @@ def oracle_moments(orc, variant, dm, d, ob, steps, seed, amp):
 
 @pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
 @pytest.mark.parametrize("mode", ["plain", "graph", "per_domain", "persistent", "plain_xcd", "persistent_xcd"])
-def test_batch_matches_oracle(orc, variant, mode):
-    print(f"\n[case] batch {variant} {mode}", flush=True)
+@pytest.mark.parametrize("scheme", ["pull", "push"])
+def test_batch_matches_oracle(orc, variant, mode, scheme):
+    print(f"\n[case] batch {variant} {mode} {scheme}", flush=True)
     doms = domains(7, 11)
     ob, pb = channel()
     steps, seed, amp = 12, 40, 2e-3
     b = P.Batch(19, variant, doms, pb)
+    b.set_scheme(2 if scheme == "pull" else 1)
     if mode.startswith("persistent"):
         b.set_tiling(128, 3)
     elif mode == "per_domain":
@@ -62,15 +64,21 @@ def test_batch_matches_oracle(orc, variant, mode):
     b.close()
 
 
-def test_batch_moving_lid(orc):
+@pytest.mark.parametrize("scheme", ["pull", "push"])
+@pytest.mark.parametrize("variant", ["f64", "f16s"])
+def test_batch_moving_lid(orc, scheme, variant):
     doms = domains(4, 3)
     ob, pb = channel(lid=(0.02, 0.0, 0.0))
-    b = P.Batch(19, "f64", doms, pb)
+    b = P.Batch(19, variant, doms, pb)
+    b.set_scheme(2 if scheme == "pull" else 1)
     b.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 5, 1e-3))
-    b.step(10)
+    b.step(4)
+    b.moments(0)  # a read in the middle of the run (pull: finishes the pending streaming)
+    b.step(3, steps_per_graph=2)
+    b.step(3)
     for d, dm in enumerate(doms):
         rho, u = b.moments(d)
-        orho, ou = oracle_moments(orc, "f64", dm, d, ob, 10, 5, 1e-3)
+        orho, ou = oracle_moments(orc, variant, dm, d, ob, 10, 5, 1e-3)
         assert np.array_equal(rho, orho) and np.array_equal(u, ou)
     b.close()
 
