@@ -230,6 +230,9 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                               /* each XCD takes a y-stripe of every z-plane (rows */
                               /* % 8 == 0, else 0); -1 (default) = 2 around       */
                               /* interior solids, else 0                          */
+#define PLB_TUNE_INDEX64 7    /* 1: the cell kernels' 64-bit index instantiations even */
+                              /* where 32-bit byte offsets fit (they run by themselves */
+                              /* for buffers over 4 GiB); default 0                    */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
  * step will use. */

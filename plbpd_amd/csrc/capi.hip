@@ -217,6 +217,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->tune_order = 0;
     s->tune_vector = -1;
     s->tune_xcd = -1;
+    s->tune_index64 = 0;
     s->post_collision = false;
     for (double& v : s->stage_ms) v = 0.0;
 

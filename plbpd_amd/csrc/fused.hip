@@ -728,8 +728,9 @@ int launch_kind(plb_state* s, KernelKind kind) {
         const dim3 grid((unsigned)ntiles), block(kBlock);
         const bool nb = s->has_solids;
         // 32-bit element indices when every population index fits
-        // 32-bit indices (and byte offsets) when every byte offset fits
-        const bool idx32 = (uint64_t)Q * s->pstride * sizeof(S) <= 0xffffffffull;
+        // 32-bit indices (and byte offsets) when every byte offset fits and
+        // 64-bit ones are not asked for
+        const bool idx32 = !s->tune_index64 && (uint64_t)Q * s->pstride * sizeof(S) <= 0xffffffffull;
         switch (kind) {
             case kPush:
                 if (nb) hipLaunchKernelGGL((k_push<Q, V, 1, true>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);

@@ -83,6 +83,7 @@ struct plb_state {
     int tune_scheme;      // PLB_SCHEME_*
     int tune_order;       // tile-row order: 0 linear, B: B x B (y, z) blocks
     int tune_xcd;         // tile mapping: 0 block i -> tile i, 1 XCD runs, 2 XCD y-stripes, -1 auto
+    int tune_index64;     // 1: 64-bit index kernels even where 32-bit offsets fit
     int tune_vector;      // row pull kernel cells per lane (1, 2, 4) where X allows; 0 off; -1 auto
     bool post_collision;  // fused: buf[cur] holds post-collision populations (pull step)
     // timers

@@ -268,7 +268,8 @@ def test_tuning_knobs_keep_results(orc, variant, solids, lid):
     combos = [{}, {T.TUNE_XCD: 1}, {T.TUNE_XCD: 2}, {T.TUNE_XCD: 2, T.TUNE_VECTOR: 0}, {T.TUNE_TILE_ORDER: 4}, {T.TUNE_TILE_ORDER: 8, T.TUNE_XCD: 1},
               {T.TUNE_BLOCK_X: 64}, {T.TUNE_NT_LOAD: 1, T.TUNE_NT_STORE: 0}, {T.TUNE_VECTOR: 0},
               {T.TUNE_VECTOR: 1}, {T.TUNE_VECTOR: 2, T.TUNE_XCD: 1}, {T.TUNE_SCHEME: 1},
-              {T.TUNE_SCHEME: 1, T.TUNE_NT_STORE: 0, T.TUNE_XCD: 1}]
+              {T.TUNE_SCHEME: 1, T.TUNE_NT_STORE: 0, T.TUNE_XCD: 1},
+              {T.TUNE_INDEX64: 1, T.TUNE_VECTOR: 0}, {T.TUNE_INDEX64: 1, T.TUNE_VECTOR: 0, T.TUNE_SCHEME: 1}]
     for knobs in combos:
         print(f"\n[case] knobs {variant} solids={solids} lid={lid} {knobs}", flush=True)
         with P.State(q, X, Y, Z, variant, P.FUSED) as st:
