@@ -117,9 +117,13 @@ typedef struct {
 } plb_init_spec;
 
 /* ---- library ------------------------------------------------------------ */
+/* Message of the last failure on the calling thread (the what() of the
+ * exception the reference would throw). */
 const char* plb_last_error(void);
+/* Library version (100 = 1.0.0).  No reference counterpart. */
 int plb_version(void);
-/* Device name, CU count, HBM bytes of the device the library would use. */
+/* Device name, CU count, HBM bytes of the device the library would use.  No
+ * reference counterpart. */
 int plb_device_info(int device, char* name, size_t name_len, int* num_cu, size_t* hbm_bytes);
 
 /* LBModel::get_directional_velocities / get_directional_weights / get_reverse
@@ -133,7 +137,7 @@ int plb_lattice_tables(int q, int32_t* c, double* w, uint32_t* rev);
  * rho and u.  A zero extent -> PLB_ERR_INVALID_ARGUMENT (Field.hpp:28-31). */
 int plb_state_create(const plb_state_desc* desc, plb_state** out);
 int plb_state_destroy(plb_state* s);
-/* Bytes of device memory the state holds. */
+/* Bytes of device memory the state holds.  No reference counterpart. */
 int plb_state_bytes(const plb_state* s, size_t* bytes);
 
 /* Host <-> device copies of SimData's fields (SimData.hpp:18-21).  In fused
@@ -142,7 +146,8 @@ int plb_state_bytes(const plb_state* s, size_t* bytes);
 int plb_upload(plb_state* s, int field, int layout, int dtype, const void* host);
 int plb_download(plb_state* s, int field, int layout, int dtype, void* host);
 /* Current populations in stored form (compact AoS, storage element type),
- * for bit-level comparison of reduced-precision storage. */
+ * for bit-level comparison of reduced-precision storage.  No reference
+ * counterpart (the reference stores float only). */
 int plb_download_raw(plb_state* s, void* host, size_t bytes);
 
 /* ---- set-up ------------------------------------------------------------- */
@@ -203,8 +208,9 @@ int plb_time_steps(plb_state* s, int nsteps, int steps_per_graph, double* ms);
 /* Copy-kernel calibration: copies `bytes` device-to-device `iters` times;
  * gbps counts bytes read plus bytes written. */
 int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms_per_copy);
-/* Fused-kernel tuning knobs.  The defaults are the settings measured fastest
- * on MI355X for D3Q19 256^3; a change drops a captured step graph. */
+/* Fused-kernel tuning knobs (no reference counterpart).  The defaults are
+ * the settings measured fastest on MI355X for D3Q19 256^3; no knob changes
+ * results; a change drops a captured step graph. */
 #define PLB_TUNE_BLOCK_X 0  /* cells per block row: 64, 128, 256 (0 = from x)   */
 #define PLB_TUNE_NT_LOAD 1  /* non-temporal population loads of the push kernel */
                             /* (0/1; default 0).  The cell pull kernel's loads   */
