@@ -21,6 +21,7 @@
 #include "Field.hpp"
 #include "Flow.hpp"
 #include "PLBPD.hpp"
+#include "plbpd_scenario.hpp"
 
 // ---- the CPU oracle's C interface (oracle/plb_oracle.cpp) ----
 extern "C" {
@@ -174,6 +175,67 @@ void test_flow_names() {
     Flow f("Poiseuille2D");
     EXPECT(f.get_num_timesteps() == 5000);
     EXPECT(f.get_lbmodel()->get_num_directions() == 9);
+    EXPECT(f.get_scenario() == nullptr);
+}
+
+// ---- scenario files (the config file the reference stubs out) ----
+std::string write_file(const std::string& name, const std::string& text) {
+    const std::string path = "/tmp/plbpd_test_" + name;
+    FILE* fp = std::fopen(path.c_str(), "w");
+    std::fputs(text.c_str(), fp);
+    std::fclose(fp);
+    return path;
+}
+
+const char* kScenario =
+    "# porous channel\n"
+    "lattice = D3Q19\n"
+    "dimensions = 24 12 10   # x y z\n"
+    "tau = 0.62\n"
+    "density = 1.0\n"
+    "force = 2e-5 0 0\n"
+    "boundary.east = periodic\nboundary.west = periodic\n"
+    "boundary.north = periodic\nboundary.south = periodic\n"
+    "boundary.up = noslip\nboundary.down = noslip\n"
+    "velocity.up = 0.02 0 0\n"
+    "timesteps = 25\n"
+    "variant = f32s\nmode = fused\n"
+    "init = perturbed 321 1e-3\n"
+    "solids.fraction = 0.1\nsolids.seed = 7\n";
+
+void test_scenario_files() {
+    const std::string path = write_file("scenario.cfg", kScenario);
+    const plbpd::Scenario sc = plbpd::read_scenario(path);
+    EXPECT(sc.lattice == "D3Q19" && sc.x == 24 && sc.y == 12 && sc.z == 10);
+    EXPECT(sc.has_tau && sc.tau == 0.62 && sc.force[0] == 2e-5);
+    EXPECT(sc.boundary_type.at("up") == "noslip" && sc.boundary_type.at("east") == "periodic");
+    EXPECT(sc.wall_velocity.at("up")[0] == 0.02);
+    EXPECT(sc.timesteps == 25 && sc.variant == PLB_F32S && sc.mode == PLB_MODE_FUSED);
+    EXPECT(sc.init.kind == PLB_INIT_PERTURBED && sc.init.seed == 321 && sc.init.amp == 1e-3);
+    const std::vector<uint8_t> m = plbpd::scenario_solids(sc);
+    size_t solid = 0;
+    for (uint8_t v : m) solid += v;
+    EXPECT(m.size() == 24 * 12 * 10 && solid > 200 && solid < 380);  // about 10% of 2880
+    EXPECT(plbpd::scenario_solids(sc) == m);                        // reproducible
+
+    Flow f(path);
+    EXPECT(f.get_scenario() != nullptr && f.get_num_timesteps() == 25);
+    EXPECT(f.get_domain()->get_relaxation_time() == 0.62);
+    EXPECT(std::get<0>(f.get_domain()->get_dimensions()) == 24);
+    const plb_boundary& b = f.get_boundary()->descriptor();
+    EXPECT(b.type[PLB_UP] == PLB_FACE_NOSLIP && b.type[PLB_EAST] == PLB_FACE_PERIODIC);
+    EXPECT(b.has_vel[PLB_DOWN] && b.vel[PLB_UP][0] == (double)0.02f);
+
+    // malformed files name the file and line
+    const std::string bad = write_file("bad.cfg", "lattice = D3Q19\ndimensions = 4 4\n");
+    EXPECT(throws<std::invalid_argument>([&] { plbpd::read_scenario(bad); }));
+    const std::string unknown = write_file("unknown.cfg", "lattice = D3Q19\ncolour = blue\n");
+    EXPECT(throws<std::invalid_argument>([&] { plbpd::read_scenario(unknown); }));
+    const std::string missing = write_file("missing.cfg", "dimensions = 4 4 4\ntau = 0.7\n");
+    EXPECT(throws<std::invalid_argument>([&] { plbpd::read_scenario(missing); }));
+    const std::string model = write_file("model.cfg", "lattice = D3Q15\ndimensions = 4 4 4\ntau = 0.7\n");
+    EXPECT(throws<std::logic_error>([&] { Flow g(model); }));  // LBModel.cpp:17
+    for (const std::string& p : {path, bad, unknown, missing, model}) std::remove(p.c_str());
 }
 
 // ------------------------------------------------------------------ GPU ---
@@ -271,6 +333,53 @@ void test_fused_march_matches_oracle(int variant, float lid) {
     EXPECT(same);
 }
 
+// A scenario file run through Flow / PLBPD on the GPU (fused f32s, moving
+// lid, interior solids) against the oracle with the same mask.
+void test_scenario_march_matches_oracle() {
+    const std::string path = write_file("scenario_gpu.cfg", kScenario);
+    Flow f(path);
+    const plbpd::Scenario& sc = *f.get_scenario();
+    plbpd::DeviceConfig cfg;
+    cfg.variant = sc.variant;
+    cfg.mode = sc.mode;
+    SimData sim(f.get_domain()->get_dimensions(), 19, cfg);
+    const std::vector<uint8_t> mask = plbpd::scenario_solids(sc);
+    f.get_boundary()->set_solid_mask(sim, mask.data());
+    PLBPD p(f.get_lbmodel(), f.get_domain(), f.get_boundary());
+    p.initialize(sim);
+    p.march_in_time(f.get_num_timesteps(), sim);
+    sim.sync_to_host();
+
+    const int X = 24, Y = 12, Z = 10;
+    const size_t n = (size_t)X * Y * Z;
+    std::vector<unsigned char> fa(19 * n * 4), fb(19 * n * 4);
+    const orc_bc ob = to_orc(f.get_boundary()->descriptor());
+    const orc_init in = {PLB_INIT_PERTURBED, 321, 1e-3};
+    const double u0[3] = {0, 0, 0};
+    orc_cor_init(PLB_F32S, 19, X, Y, Z, 1.0, u0, &in, fa.data());
+    const int which = orc_cor_run(PLB_F32S, 19, X, Y, Z, 0.62, sc.force.data(), &ob, mask.data(),
+                                  (int)f.get_num_timesteps(), fa.data(), fb.data());
+    std::vector<double> rho(n), u(3 * n);
+    orc_cor_moments(PLB_F32S, 19, X, Y, Z, &ob, mask.data(), which == 0 ? fa.data() : fb.data(), rho.data(),
+                    u.data());
+    std::vector<uint8_t> flags(n);
+    orc_cor_flags(19, X, Y, Z, &ob, mask.data(), flags.data());
+    bool same = true;
+    size_t fluid = 0;
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) {
+                const size_t c = ((size_t)z * Y + y) * X + x;
+                if (flags[c]) continue;
+                ++fluid;
+                same &= sim.rho->at(z + 1, y + 1, x + 1) == rho[c];
+                for (int i = 0; i < 3; ++i) same &= sim.u->at(z + 1, y + 1, x + 1, i) == u[3 * c + i];
+            }
+    EXPECT(fluid > 1500);
+    EXPECT(same);
+    std::remove(path.c_str());
+}
+
 void test_device_errors() {
     Setup s = couette(6, 5, 5, 0.8, {0, 0, 0}, 0.0f);
     plbpd::DeviceConfig cfg;
@@ -315,11 +424,13 @@ int main(int argc, char** argv) {
         test_domain();
         test_eqlbdist();
         test_flow_names();
+        test_scenario_files();
     }
     if (what == "gpu" || what == "all") {
         test_literal_march_matches_oracle();
         for (int v : {PLB_F64, PLB_F32S, PLB_F16S}) test_fused_march_matches_oracle(v, 0.0f);
         for (int v : {PLB_F64, PLB_F32S}) test_fused_march_matches_oracle(v, 0.03f);  // moving lid
+        test_scenario_march_matches_oracle();
         test_device_errors();
         test_state_file_resume();
     }

@@ -35,3 +35,21 @@ def test_runner(tmp_path, args):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Time taken by timeloop" in r.stdout
     assert out.exists() and out.stat().st_size > 0
+
+
+@pytest.mark.gpu
+def test_runner_scenario_file(tmp_path):
+    """plbpd_run with a scenario file instead of a preset name: lattice,
+    boundary, storage, step semantics, initial state and solids from text."""
+    cfg = tmp_path / "porous.cfg"
+    cfg.write_text("lattice = D3Q19\ndimensions = 32 16 12\nviscosity = 0.05\nforce = 1e-5 0 0\n"
+                   "boundary.east = periodic\nboundary.west = periodic\n"
+                   "boundary.north = periodic\nboundary.south = periodic\n"
+                   "boundary.up = noslip\nboundary.down = noslip\nvelocity.up = 0.01 0 0\n"
+                   "timesteps = 40\nvariant = f16s\nmode = fused\ninit = perturbed 5 1e-3\n"
+                   "solids.fraction = 0.05\nsolids.seed = 3\n")
+    out = tmp_path / "state.h5"
+    r = subprocess.run([RUN, str(cfg), "--out", str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Time taken by timeloop" in r.stdout
+    assert out.exists() and out.stat().st_size > 0
