@@ -356,6 +356,9 @@ int plb_set_boundary(plb_state* s, const plb_boundary* bc, const uint8_t* mask) 
         s->bc_set = true;
         return PLB_OK;
     }
+    // a pull-scheme state finishes its pending streaming with the old flags
+    rc = fused_to_pre(s);
+    if (rc) return rc;
     std::vector<uint8_t>& m = s->mask_host;
     if (mask)
         m.assign(mask, mask + s->ncell);

@@ -587,6 +587,12 @@ template void make_fused_bc<float>(const plb_boundary&, int, FusedBC<float>*);
 int fused_build_flags(plb_state* s, const plb_boundary* bc, const uint8_t* mask) {
     int rc = check_fused_bc(s->q, s->X, s->Y, s->Z, bc);
     if (rc) return rc;
+    // Populations between pull steps still owe the streaming of the last
+    // step; it streams through the walls and solids of that step, so it is
+    // finished with the old flags before they change (as the push step, which
+    // streams within the step, would have).
+    rc = fused_to_pre(s);
+    if (rc) return rc;
     const uint64_t n = s->ncell;
     std::vector<uint8_t> fl(n);
     bool any_solid = false;

@@ -286,6 +286,39 @@ def test_tuning_knobs_keep_results(orc, variant, solids, lid):
             assert np.array_equal(raw[fluid], o["f"][fluid]), knobs
 
 
+@pytest.mark.parametrize("variant", ["f64", "f32s"])
+def test_boundary_change_mid_run(orc, variant):
+    """Solids are added and the lid velocity changes between steps: the pull step (whose
+    state owes the last step's streaming) gives the push step's result and
+    the oracle's, the pending streaming using the old walls and solids."""
+    q, X, Y, Z = 19, 32, 12, 10
+    ob1, pb1 = fused_bc(q, True, lid=(0.02, 0.0, 0.0))
+    ob2, pb2 = fused_bc(q, True, lid=(-0.01, 0.01, 0.0))
+    rng = np.random.default_rng(12)
+    m1 = (rng.random((Z, Y, X)) < 0.08).astype(np.uint8)
+    # solids are only added (a cell that turns fluid would have no populations)
+    m2 = m1 | (rng.random((Z, Y, X)) < 0.05).astype(np.uint8)
+    init, tau, F = (P.INIT_PERTURBED, 31, 1e-3), 0.7, (1e-5, 0, 0)
+    o = orc.cor_state(variant, q, X, Y, Z)
+    orc.cor_init(o, 1.0, (0, 0, 0), init)
+    orc.cor_run(o, tau, F, ob1, 7, m1)
+    orc.cor_run(o, tau, F, ob2, 6, m2)
+    fluid = orc.cor_flags(q, X, Y, Z, ob2, m2) == 0
+    for scheme in (2, 1):
+        print(f"\n[case] boundary change {variant} scheme={scheme}", flush=True)
+        with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+            st.set_tuning(P.capi.TUNE_SCHEME, scheme)
+            st.set_boundary(pb1, m1)
+            st.initialize(1.0, (0, 0, 0), init, pb1)
+            st.configure(tau, F, pb1)
+            st.step(7)
+            st.set_boundary(pb2, m2)
+            st.configure(tau, F, pb2)
+            st.step(6)
+            raw = st.download_raw()
+        assert np.array_equal(raw[fluid], o["f"][fluid]), scheme
+
+
 @pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
 def test_pull_push_same_trajectory_with_midrun_reads(variant):
     """Pull and push steps give identical populations, and reading the state
