@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "bgk.hpp"
+#include "fastdiv.hpp"
 #include "state.hpp"
 
 struct plb_batch;
@@ -30,10 +31,20 @@ constexpr int kMaxBlock = 256;
 template <class T>
 struct BDom {
     uint32_t X, Y, Z, N;
-    uint64_t off;  // element offset of population 0 in the pool
-    uint64_t ps;   // elements between populations
+    FastDiv dX, dY;  // division by X and Y (cell -> coordinates)
+    uint64_t off;    // element offset of population 0 in the pool
+    uint64_t ps;     // elements between populations
     T tau, om, omm, F0, F1, F2;
 };
+
+// Coordinates of cell c of a domain.
+template <class T>
+__device__ inline void cell_xyz(const BDom<T>& D, uint32_t c, uint32_t& x, uint32_t& y, uint32_t& z) {
+    const uint32_t r = fdiv(c, D.dX);
+    x = c - r * D.X;
+    z = fdiv(r, D.dY);
+    y = r - z * D.Y;
+}
 
 template <class T>
 struct BArgs {
@@ -57,10 +68,8 @@ __device__ inline void batch_cell(const typename Prec<V>::S* __restrict__ A, typ
                                   uint32_t c) {
     using P = Prec<V>;
     using T = typename P::T;
-    const uint32_t x = c % D.X;
-    const uint32_t r = c / D.X;
-    const uint32_t y = r % D.Y;
-    const uint32_t z = r / D.Y;
+    uint32_t x, y, z;
+    cell_xyz(D, c, x, y, z);
     // wall layers of no-slip faces are solid
     const bool solid_self = (bc.wall_lo[0] && x == 0) || (bc.wall_hi[0] && x == D.X - 1) ||
                             (bc.wall_lo[1] && y == 0) || (bc.wall_hi[1] && y == D.Y - 1) ||
@@ -125,10 +134,8 @@ __device__ inline void batch_pull_cell(const typename Prec<V>::S* __restrict__ A
                                        const FusedBC<typename Prec<V>::T>& bc, uint32_t c) {
     using P = Prec<V>;
     using T = typename P::T;
-    const uint32_t x = c % D.X;
-    const uint32_t r = c / D.X;
-    const uint32_t y = r % D.Y;
-    const uint32_t z = r / D.Y;
+    uint32_t x, y, z;
+    cell_xyz(D, c, x, y, z);
     const typename P::S* a = A + D.off;
     typename P::S* b = B + D.off;
     const bool solid_self = (bc.wall_lo[0] && x == 0) || (bc.wall_hi[0] && x == D.X - 1) ||
@@ -180,10 +187,8 @@ __device__ inline void batch_collide_cell(typename Prec<V>::S* A, const BDom<typ
                                           const FusedBC<typename Prec<V>::T>& bc, uint32_t c) {
     using P = Prec<V>;
     using T = typename P::T;
-    const uint32_t x = c % D.X;
-    const uint32_t r = c / D.X;
-    const uint32_t y = r % D.Y;
-    const uint32_t z = r / D.Y;
+    uint32_t x, y, z;
+    cell_xyz(D, c, x, y, z);
     const bool solid_self = (bc.wall_lo[0] && x == 0) || (bc.wall_hi[0] && x == D.X - 1) ||
                             (bc.wall_lo[1] && y == 0) || (bc.wall_hi[1] && y == D.Y - 1) ||
                             (bc.wall_lo[2] && z == 0) || (bc.wall_hi[2] && z == D.Z - 1);
@@ -419,6 +424,8 @@ int upload_doms(plb_batch* b) {
         o.Y = s.y;
         o.Z = s.z;
         o.N = (uint32_t)((uint64_t)s.x * s.y * s.z);
+        o.dX = make_fastdiv((uint32_t)s.x);
+        o.dY = make_fastdiv((uint32_t)s.y);
         o.off = b->off[d];
         o.ps = b->ps[d];
         o.tau = (T)s.tau;
