@@ -386,9 +386,15 @@ int plb_stream(plb_state* s, const char* stream_type) {
     int rc = enter(s);
     if (rc) return rc;
     const std::string t = stream_type ? stream_type : "";
-    if (t == "pull") return fail(PLB_ERR_LOGIC, "Streaming::_pull_tbb has not yet been implemented");
-    if (t != "push") return fail(PLB_ERR_LOGIC, "Invalid stream type: " + t);
-    if (s->mode == PLB_MODE_LITERAL) return timed(s, PLB_STAGE_STREAM, [&] { return lit_push(s); });
+    if (t != "push" && t != "pull") return fail(PLB_ERR_LOGIC, "Invalid stream type: " + t);
+    if (s->mode == PLB_MODE_LITERAL) {
+        if (t == "pull") return fail(PLB_ERR_LOGIC, "Streaming::_pull_tbb has not yet been implemented");
+        return timed(s, PLB_STAGE_STREAM, [&] { return lit_push(s); });
+    }
+    // Fused mode: both schemes give the same numbers; "pull" asks for the
+    // pull kernels, which the push scheme knob would contradict.
+    if (t == "pull" && !fused_uses_pull(s))
+        return fail(PLB_ERR_LOGIC, "stream type pull with the push scheme selected (PLB_TUNE_SCHEME)");
     const bool collide = s->collide_pending;
     s->collide_pending = false;
     return timed(s, PLB_STAGE_FUSED, [&] { return fused_launch(s, collide); });

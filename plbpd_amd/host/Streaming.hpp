@@ -1,7 +1,8 @@
 // Drop-in Streaming (interface of the reference's Streaming.hpp:10-34):
 // "push" streams and swaps n / ntmp on the device; "pull" and any other
 // string throw std::logic_error as in the reference (Streaming.cpp:18-24,
-// 96-102).
+// 96-102).  With the fused device mode, "pull" runs the fused pull step (the
+// same numbers as "push").
 #ifndef PLBPD_AMD_HOST_STREAMING_HPP_
 #define PLBPD_AMD_HOST_STREAMING_HPP_
 

@@ -387,6 +387,20 @@ void test_device_errors() {
     Streaming st(s.model.get());
     EXPECT(throws<std::logic_error>([&] { st(sim, "pull"); }));
     EXPECT(throws<std::logic_error>([&] { st(sim, "sideways"); }));
+    // A fused state streams "pull" (the pull step), and still refuses others.
+    plbpd::DeviceConfig fcfg;
+    fcfg.mode = PLB_MODE_FUSED;
+    SimData fsim(s.domain->get_dimensions(), 19, fcfg);
+    PLBPD p(s.model.get(), s.domain.get(), s.boundary.get());
+    p.initialize(fsim);
+    bool pulled = true;
+    try {
+        st(fsim, "pull");
+    } catch (...) {
+        pulled = false;
+    }
+    EXPECT(pulled);
+    EXPECT(throws<std::logic_error>([&] { st(fsim, "sideways"); }));
     BoundaryType type = {{"up", "noslip"}, {"down", "noslip"}};
     BoundaryVelocity vel = {{"up", {0.0f, 0.0f, 0.0f}}};
     Boundary bad(s.model.get(), s.domain.get(), 1.0f, type, vel);

@@ -347,9 +347,12 @@ def test_pull_push_same_trajectory_with_midrun_reads(variant):
 
 
 @pytest.mark.parametrize("variant", ["f64", "f32s"])
-def test_dropin_call_sequence_fused(orc, variant):
+@pytest.mark.parametrize("stream_type", ["push", "pull"])
+def test_dropin_call_sequence_fused(orc, variant, stream_type):
     """The reference call sequence (collide, stream, noslip, periodicity,
-    moments, reset) on a fused state equals plb_step."""
+    moments, reset) on a fused state equals plb_step, with either stream type
+    (a fused state streams "pull", which the reference leaves unimplemented)."""
+    print(f"\n[case] fused call sequence {variant} stream={stream_type}", flush=True)
     q, X, Y, Z = 19, 16, 8, 10
     ob, pb = fused_bc(q, True)
     tau, F = 0.7, (1e-5, 0, 0)
@@ -359,13 +362,16 @@ def test_dropin_call_sequence_fused(orc, variant):
         b.configure(tau, F, pb)
         for _ in range(6):
             a.collide(tau, F)
-            a.stream("push")
+            a.stream(stream_type)
             a.apply_noslip(pb)
             a.apply_periodicity(pb)
             a.calc_moments()
             a.reset(pb)
         b.step(6)
         assert np.array_equal(a.download_raw(), b.download_raw())
+        a.set_tuning(P.capi.TUNE_SCHEME, SCHEMES["push"])
+        with pytest.raises(P.LogicError):  # "pull" contradicts the push scheme knob
+            a.stream("pull")
 
 
 def test_graph_replay_matches_eager():
