@@ -6,11 +6,14 @@
 //   f[off_d + k * ps_d + c],  c = (z*Y_d + y)*X_d + x,  ps_d = N_d rounded up to 64.
 // Work: a tile is `tile` consecutive cells of one domain (the last tile of a
 // domain may be short).  A launch covers every tile, either one block per
-// tile or a persistent grid in which each block walks a contiguous run of
-// tiles; tile runs are handed to XCDs contiguously so a domain's neighbouring
-// rows share an L2.  The per-cell work is the push step of fused.hip: the
-// same collision, periodic wraps on periodic axes and halfway bounce-back on
-// the wall layers of no-slip faces (stationary or moving).
+// tile or a persistent grid whose blocks walk the tiles.  By default
+// workgroup i takes tile i (a persistent block strides by the grid), so the
+// chip sweeps one window of memory; the XCD mapping instead hands each XCD a
+// contiguous run of tiles, so a domain's neighbouring rows share an L2
+// (batch_tiles).  The per-cell work is the step of fused.hip, pull (the
+// default) or push: the same collision, periodic wraps on periodic axes and
+// halfway bounce-back on the wall layers of no-slip faces (stationary or
+// moving).
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -775,7 +778,8 @@ int plb_batch_autotune(plb_batch* b, int steps, double* results, int max_results
     PLB_HIP_TRY(hipEventCreate(&e0));
     PLB_HIP_TRY(hipEventCreate(&e1));
     int rc = PLB_OK;
-    // every candidate runs an even number of steps, so the buffers end where they began
+    // every candidate runs an even number of steps (the total the caller sees
+    // is even; the buffer parity is tracked in b->cur either way)
     const int even = steps + (steps & 1);
     for (int t : tiles) {
         for (int g : grids) {
