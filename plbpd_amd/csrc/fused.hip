@@ -1,16 +1,20 @@
 // Fused mode: one kernel per time step does the work of CollisionSRT,
-// Streaming (push), Boundary::apply_noslip / apply_periodicity and the
-// moments that the next collision needs (CollisionSRT.cpp:153-176,
-// Streaming.cpp:63-74, Boundary.cpp:189-281, CalcMoments.hpp:22-37), without
-// the stale seam reads of the staged step.
+// Streaming, Boundary::apply_noslip / apply_periodicity and the moments that
+// the next collision needs (CollisionSRT.cpp:153-176, Streaming.cpp:63-74,
+// Boundary.cpp:189-281, CalcMoments.hpp:22-37), without the stale seam reads
+// of the staged step.
 //
 // Layout in HBM (structure of arrays, no halo):
 //   f[k * pstride + c],  c = (z*Y + y)*X + x,  x fastest.
-// One thread per cell; a block covers TX x-consecutive cells of TY rows, so a
-// wavefront reads 64 consecutive cells of one population (coalesced) and
-// writes them shifted by the direction's offset.  Periodic axes wrap by index
-// arithmetic; a neighbour that is a wall or solid reflects the population
-// into the cell's own reverse slot (halfway bounce-back, with the moving-wall
+// Two schemes give the same numbers.  The pull step (the default; k_pull here
+// and the row kernel of fused_row.hip) gathers each population from x - c_k
+// and writes the cell's own slots, so every store is a whole aligned line.
+// The push step (k_push) reads the cell's own slots and writes each
+// population at x + c_k.  One thread per cell in both cell kernels: a block
+// covers TX x-consecutive cells of TY rows, so a wavefront touches 64
+// consecutive cells of one population.  Periodic axes wrap by index
+// arithmetic; a neighbour that is a wall or solid sends the population back
+// to the cell's own reverse slot (halfway bounce-back, with the moving-wall
 // term of Boundary.cpp:213-217).
 #include <type_traits>
 #include <vector>
@@ -711,8 +715,8 @@ int launch_kind(plb_state* s, KernelKind kind) {
         a.dnby = make_fastdiv(a.order ? nty / a.order : 1u);
         // Automatic tile mapping: around interior solids the bounce-back loads
         // read lines again that the y +- 1 and z +- 1 rows read; XCD y-stripes
-        // keep those rows in one L2 (HBM reads 1.08x the algorithmic volume
-        // instead of 1.55x with tile i on workgroup i, at the same speed).
+        // keep those rows in one L2 (for the cell kernel, HBM reads fell from
+        // 1.55x to 1.08x the algorithmic volume, at the same speed).
         int xm = s->tune_xcd >= 0 ? s->tune_xcd : (s->has_solids ? 2 : 0);
         if (xm == 2 && nty % 8 != 0) xm = 0;
         a.xcd = (uint32_t)xm;
@@ -733,7 +737,6 @@ int launch_kind(plb_state* s, KernelKind kind) {
         S* B = (S*)s->buf[s->cur ^ 1];
         const dim3 grid((unsigned)ntiles), block(kBlock);
         const bool nb = s->has_solids;
-        // 32-bit element indices when every population index fits
         // 32-bit indices (and byte offsets) when every byte offset fits and
         // 64-bit ones are not asked for
         const bool idx32 = !s->tune_index64 && (uint64_t)Q * s->pstride * sizeof(S) <= 0xffffffffull;
