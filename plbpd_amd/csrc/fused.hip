@@ -304,8 +304,8 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
     if (COLLIDE) {
         const T rho = bgk<Q, P>(f, a, true);
         // MOV: an instantiation for boundaries with a moving wall, so that
-        // the others carry no wall-term code (it costs 2-byte storage, whose
-        // step is issue-bound, about 5%)
+        // the others carry no wall-term code (compiled in everywhere, the
+        // dormant branch cost the 2-byte step about 5%)
         if (MOV && a.bc.moving && !self_solid) wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
     }
     own_store<Q, V, NTS>(f, B, c, ps, self_solid);
@@ -661,9 +661,8 @@ int launch_kind(plb_state* s, KernelKind kind) {
     // the row pull kernel (fused_row.hip) takes rows of 64 * vec * (1, 2 or 4)
     // cells.  Automatic choice, from measurement on MI355X (D3Q19 256^3): for
     // 4- and 8-byte storage 1 cell per lane without interior solids, and
-    // around solids 4 (4-byte) or 2 (8-byte); for 2-byte storage, whose step
-    // is bound by instruction issue rather than HBM, the cell-per-thread
-    // kernel, which needs fewer instructions per cell.
+    // around solids 4 (4-byte) or 2 (8-byte); for 2-byte storage the
+    // cell-per-thread kernel, which measured faster than every row width.
     int vec = kind == kPull ? s->tune_vector : 0;
     if (vec < 0) {
         if (s->sbytes == 2)

@@ -330,6 +330,7 @@ void test_fused_march_matches_oracle(int variant, float lid) {
                 same &= sim.rho->at(z + 1, y + 1, x + 1) == rho[c];
                 for (int i = 0; i < 3; ++i) same &= sim.u->at(z + 1, y + 1, x + 1, i) == u[3 * c + i];
             }
+    if (!same) std::fprintf(stderr, "fused march differs from the oracle: variant %d, lid %g\n", variant, lid);
     EXPECT(same);
 }
 
