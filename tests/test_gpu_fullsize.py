@@ -79,3 +79,38 @@ def test_c2_parity(orc, mode):
             st.step(steps)
             rho, u = st.moments()
         assert np.array_equal(rho, orho) and np.array_equal(u, ou)
+
+
+@pytest.mark.parametrize("X", [64, 128])  # cell kernel; row kernel (2 cells per lane)
+def test_past_4gib_index_range(X):
+    """A domain whose population buffers pass 2^32 bytes takes the 64-bit
+    index kernels.  Interior solids repeating every 12 z-planes (12 does not
+    divide the 2^17 planes of a 4 GiB address wrap), a uniform start and a
+    body force give a flow with the same period, so after 20 steps every
+    12-plane slab must equal, bit for bit, a 12-plane domain that takes the
+    32-bit kernels."""
+    Y, period, steps = 64, 12, 20
+    Z = period * ((28311552 // (X * Y)) // period)
+    q, esize = 19, 8
+    assert q * X * Y * Z * esize > 2**32  # past the 32-bit byte-offset range
+    rng = np.random.default_rng(12)
+    tile = (rng.random((period, Y, X)) < 0.1).astype(np.uint8)
+    # walls on the y faces; x and z periodic
+    pb = P.boundary([P.PERIODIC, P.PERIODIC, P.NOSLIP, P.NOSLIP, P.PERIODIC, P.PERIODIC],
+                    {P.NORTH: (0.0, 0.0, 0.0), P.SOUTH: (0.0, 0.0, 0.0)})
+    print(f"\n[case] past 4 GiB: D3Q19 f64 {X}x{Y}x{Z}, solids period {period}", flush=True)
+
+    def run(nz, mask):
+        with P.State(q, X, Y, nz, "f64", P.FUSED) as st:
+            st.set_boundary(pb, mask)
+            st.initialize(1.0, (0.01, 0, 0), None, pb)
+            st.configure(0.6, (2e-6, 0, 0), pb)
+            st.step(steps)
+            return st.moments()
+
+    rho_s, u_s = run(period, tile)
+    rho_b, u_b = run(Z, np.tile(tile, (Z // period, 1, 1)))
+    reps = Z // period
+    assert np.array_equal(rho_b.reshape(reps, -1), np.broadcast_to(rho_s.reshape(1, -1), (reps, rho_s.size)))
+    assert np.array_equal(u_b.reshape(reps, -1), np.broadcast_to(u_s.reshape(1, -1), (reps, u_s.size)))
+    assert np.abs(u_s).max() > 1e-6  # the flow really moved
