@@ -7,7 +7,7 @@
 // Layout in HBM (structure of arrays, no halo):
 //   f[k * pstride + c],  c = (z*Y + y)*X + x,  x fastest.
 // Two schemes give the same numbers.  The pull step (the default; k_pull here
-// and the row kernel of fused_row.hip) gathers each population from x - c_k
+// and the row kernel of fused_row.hpp) gathers each population from x - c_k
 // and writes the cell's own slots, so every store is a whole aligned line.
 // The push step (k_push) reads the cell's own slots and writes each
 // population at x + c_k.  One thread per cell in both cell kernels: a block
@@ -658,23 +658,24 @@ enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly };
 // Launch geometry and arguments shared by every fused-layout kernel.
 int launch_kind(plb_state* s, KernelKind kind) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
-    // the row pull kernel (fused_row.hip) takes rows of 64 * vec * (1, 2 or 4)
-    // cells.  Automatic choice, from measurement on MI355X (D3Q19 256^3): for
-    // 4- and 8-byte storage 1 cell per lane without interior solids, and
-    // around solids 4 (4-byte) or 2 (8-byte); for 2-byte storage the
-    // cell-per-thread kernel, which measured faster than every row width.
-    int vec = kind == kPull ? s->tune_vector : 0;
-    if (vec < 0) {
-        if (s->sbytes == 2)
-            vec = 0;
-        else if (s->has_solids)
-            vec = s->sbytes == 4 ? 4 : 2;
-        else
-            vec = 1;
+    // the row pull kernel (fused_row.hpp) takes rows of 64 * vec * wpr cells
+    // (kRowShapes).  Automatic choice, from measurement on MI355X (D3Q19
+    // 256^3): for 4- and 8-byte storage 1 cell per lane without interior
+    // solids, and around solids 4 (4-byte) or 2 (8-byte), else the next width
+    // with a kernel for X; for 2-byte storage the cell-per-thread kernel,
+    // which measured faster than every row width.
+    auto row_ok = [&](int v) { return v > 0 && s->pstride % (size_t)v == 0 && pull_row_rows(v, s->X) > 0; };
+    int vec = 0;
+    if (kind == kPull && s->tune_vector >= 0) {
+        vec = row_ok(s->tune_vector) ? s->tune_vector : 0;
+    } else if (kind == kPull && s->sbytes != 2) {
+        static const int kSolid4[] = {4, 2, 1}, kSolid8[] = {2, 1}, kOpen[] = {1, 2};
+        const int* pref = s->has_solids ? (s->sbytes == 4 ? kSolid4 : kSolid8) : kOpen;
+        const int npref = s->has_solids ? (s->sbytes == 4 ? 3 : 2) : 2;
+        for (int i = 0; i < npref && !vec; ++i)
+            if (row_ok(pref[i])) vec = pref[i];
     }
-    if (vec && s->pstride % (size_t)vec) vec = 0;  // vec-wide accesses must stay aligned
     const int row_rows = vec ? pull_row_rows(vec, s->X) : 0;
-    if (!row_rows) vec = 0;
     unsigned tx = 64;
     if (vec)
         tx = 256;
@@ -750,7 +751,9 @@ int launch_kind(plb_state* s, KernelKind kind) {
                 break;
             case kPull:
                 if (vec) {
-                    launch_pull_row<Q, V>(a, vec, a.nt_store != 0, nb, A, B, s->flags, s->nbmask, grid.x, s->stream);
+                    if (!launch_pull_row<Q, V>(a, vec, a.nt_store != 0, nb, A, B, s->flags, s->nbmask, grid.x,
+                                               s->stream))
+                        return fail(PLB_ERR_LOGIC, "no row pull kernel for this row length");
                 } else if (idx32) {
                     // non-temporal stores and moving walls by instantiation on the hot path
                     auto go = [&](auto nbc, auto ntc, auto mvc) {

@@ -1,5 +1,5 @@
 // Device-side pieces shared by the fused-layout kernels (fused.hip and
-// fused_row.hip): launch arguments, the block -> cell mapping and the BGK
+// fused_row.hpp): launch arguments, the block -> cell mapping and the BGK
 // update.  Layout in HBM (structure of arrays, no halo):
 //   f[k * pstride + c],  c = (z*Y + y)*X + x,  x fastest.
 #pragma once
@@ -106,14 +106,27 @@ __device__ inline typename P::T bgk(typename P::T (&f)[Q], const FusedArgs<typen
 
 }  // namespace
 
-// Row pull step (fused_row.hip): rows of X == 64 * vec * wpr cells, vec cells
-// per lane, wpr wavefronts per row; nts = non-temporal stores.  Returns false
-// if no kernel exists for (vec, X).
+// Row pull step (fused_row.hpp): rows of X == 64 * vec * wpr cells, vec cells
+// per lane, wpr wavefronts per row.  The shapes it is built for (the launcher
+// in fused_row.hpp has one branch per entry):
+struct RowShape {
+    int vec, wpr;
+};
+constexpr RowShape kRowShapes[] = {{1, 1}, {1, 2}, {1, 4}, {1, 6}, {1, 8}, {2, 1},
+                                   {2, 2}, {2, 3}, {2, 4}, {4, 1}, {4, 2}};
+// Wavefronts per block: 4 rows of one wavefront, 2 rows of two, else one row.
+__host__ __device__ constexpr uint32_t row_block_waves(int wpr) { return wpr >= 3 ? (uint32_t)wpr : 4u; }
+// Rows per block of the row pull step for (vec, X), or 0 where it has no kernel.
+inline int pull_row_rows(int vec, int X) {
+    for (const RowShape& r : kRowShapes)
+        if (r.vec == vec && X == 64 * r.vec * r.wpr) return (int)row_block_waves(r.wpr) / r.wpr;
+    return 0;
+}
+// The row pull step; nts = non-temporal stores.  Returns false if no kernel
+// exists for (vec, X).
 template <int Q, int V>
 bool launch_pull_row(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts, bool nb,
                      const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
                      const uint32_t* nbmask, unsigned nblocks, hipStream_t stream);
-// Rows per block of the row pull step for (vec, X), or 0 where it has no kernel.
-int pull_row_rows(int vec, int X);
 
 }  // namespace plb

@@ -6,13 +6,18 @@
 //
 // An x-row of X = 64 * VEC * WPR cells is cut into WPR wavefront segments;
 // lane l of segment s owns cells x0 .. x0 + VEC - 1, x0 = (64 s + l) VEC.
-// Each population row (y - c_y, z - c_z) is loaded unshifted, so every cache
-// line is read by exactly one wavefront (which is what lets these loads be
-// non-temporal where there are no interior solids).  The x shift is done in registers: inside a lane by moving
-// values between its VEC cells, across lanes by a rotate, and across
-// segments through LDS.  The segments of a row wrap around, which is the
-// periodic x seam; a walled x axis has solid wall cells there, whose values
-// bounce-back replaces.
+// A block takes 4 rows of one segment, 2 rows of two segments, or one row of
+// three or more segments (row_block_waves).  Each population row
+// (y - c_y, z - c_z) is loaded unshifted, so every cache line is read by
+// exactly one wavefront (which is what lets these loads be non-temporal where
+// there are no interior solids).  The x shift is done in registers: inside a
+// lane by moving values between its VEC cells, across lanes by a rotate, and
+// across segments through LDS.  The segments of a row wrap around, which is
+// the periodic x seam; a walled x axis has solid wall cells there, whose
+// values bounce-back replaces.  Each lattice instantiates these templates in
+// its own file (fused_row_q9.hip, fused_row_q19.hip, fused_row_q27.hip).
+#pragma once
+
 #include <type_traits>
 
 #include "fused_kernels.hpp"
@@ -90,7 +95,7 @@ __device__ inline void row_wall_terms(typename P::T (&f)[Q], typename P::T rho, 
 }
 
 template <int Q, int V, int VEC, int WPR, int NB, bool NTS>
-__global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* __restrict__ A,
+__global__ __launch_bounds__(64 * row_block_waves(WPR)) void k_pull_row(const typename Prec<V>::S* __restrict__ A,
                                                      typename Prec<V>::S* __restrict__ B,
                                                      const uint8_t* __restrict__ flags,
                                                      const uint32_t* __restrict__ nbmask,
@@ -98,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_row(const typename Prec<V>::S* 
     using P = Prec<V>;
     using T = typename P::T;
     using S = typename P::S;
-    constexpr uint32_t kWaves = kBlock / 64;
+    constexpr uint32_t kWaves = row_block_waves(WPR);
     constexpr uint32_t kRows = kWaves / WPR;
     constexpr int NX = nxdirs<Q>();
     // segment edges: first value (for c_x < 0) and last value (c_x > 0)
@@ -260,7 +265,7 @@ template <int Q, int V, int VEC, int WPR>
 void launch_one(const FusedArgs<typename Prec<V>::T>& a, bool nts, bool nb, const typename Prec<V>::S* A,
                 typename Prec<V>::S* B, const uint8_t* flags, const uint32_t* nbmask, unsigned nblocks,
                 hipStream_t stream) {
-    const dim3 grid(nblocks), block(kBlock);
+    const dim3 grid(nblocks), block(64 * row_block_waves(WPR));
 #define PLB_ROW(NBv, NTv)                                                                                   \
     hipLaunchKernelGGL((k_pull_row<Q, V, VEC, WPR, NBv, NTv>), grid, block, 0, stream, A, B, flags, nbmask, a)
     if (nb) {
@@ -275,36 +280,29 @@ void launch_one(const FusedArgs<typename Prec<V>::T>& a, bool nts, bool nb, cons
 
 }  // namespace
 
+// The row pull step for (vec, X), one branch per entry of kRowShapes.
 template <int Q, int V>
 bool launch_pull_row(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts, bool nb,
                      const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
                      const uint32_t* nbmask, unsigned nblocks, hipStream_t stream) {
     const uint32_t X = a.X;
-    if (vec == 1 && X == 64) launch_one<Q, V, 1, 1>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
-    else if (vec == 1 && X == 128) launch_one<Q, V, 1, 2>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
-    else if (vec == 1 && X == 256) launch_one<Q, V, 1, 4>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
-    else if (vec == 2 && X == 128) launch_one<Q, V, 2, 1>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
-    else if (vec == 2 && X == 256) launch_one<Q, V, 2, 2>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
-    else if (vec == 4 && X == 256) launch_one<Q, V, 4, 1>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
-    else return false;
-    return true;
+#define PLB_SHAPE(VECv, WPRv)                                                                      \
+    if (vec == VECv && X == 64u * VECv * WPRv) {                                                   \
+        launch_one<Q, V, VECv, WPRv>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);            \
+        return true;                                                                               \
+    }
+    PLB_SHAPE(1, 1) PLB_SHAPE(1, 2) PLB_SHAPE(1, 4) PLB_SHAPE(1, 6) PLB_SHAPE(1, 8)
+    PLB_SHAPE(2, 1) PLB_SHAPE(2, 2) PLB_SHAPE(2, 3) PLB_SHAPE(2, 4)
+    PLB_SHAPE(4, 1) PLB_SHAPE(4, 2)
+#undef PLB_SHAPE
+    return false;
 }
 
-// Rows per block for (vec, X), or 0 where launch_pull_row has no kernel.
-int pull_row_rows(int vec, int X) {
-    if (vec == 1 && (X == 64 || X == 128 || X == 256)) return 4 / (X / 64);
-    if (vec == 2 && (X == 128 || X == 256)) return 4 / (X / 128);
-    if (vec == 4 && X == 256) return 4;
-    return 0;
-}
-
-#define PLB_INST(Q, V)                                                                                      \
+#define PLB_ROW_INST(Q, V)                                                                                  \
     template bool launch_pull_row<Q, V>(const FusedArgs<typename Prec<V>::T>&, int, bool, bool,            \
                                         const typename Prec<V>::S*, typename Prec<V>::S*, const uint8_t*,  \
                                         const uint32_t*, unsigned, hipStream_t);
-PLB_INST(9, kF64) PLB_INST(9, kF32) PLB_INST(9, kF64A32) PLB_INST(9, kF32S) PLB_INST(9, kF16S)
-PLB_INST(19, kF64) PLB_INST(19, kF32) PLB_INST(19, kF64A32) PLB_INST(19, kF32S) PLB_INST(19, kF16S)
-PLB_INST(27, kF64) PLB_INST(27, kF32) PLB_INST(27, kF64A32) PLB_INST(27, kF32S) PLB_INST(27, kF16S)
-#undef PLB_INST
+#define PLB_ROW_INST_Q(Q) \
+    PLB_ROW_INST(Q, kF64) PLB_ROW_INST(Q, kF32) PLB_ROW_INST(Q, kF64A32) PLB_ROW_INST(Q, kF32S) PLB_ROW_INST(Q, kF16S)
 
 }  // namespace plb

@@ -160,10 +160,11 @@ FUSED_CASES = [  # q, X, Y, Z, walls?, solids?
     (19, 12, 10, 8, False, False),
     (27, 16, 10, 9, True, False),
     (27, 10, 9, 8, True, True),
-    # rows of 64, 128 or 256 cells also run the row pull kernel (one cell per lane)
+    # rows of 128 to 512 cells also run the row pull kernel (one cell per lane)
     (19, 256, 4, 6, True, True),
     (19, 128, 5, 7, False, False),
     (27, 128, 4, 5, True, True),
+    (27, 384, 3, 4, True, False),
     (9, 256, 1, 8, True, False),
 ]
 SCHEMES = {"push": 1, "pull": 2}
@@ -196,7 +197,7 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
     orc.cor_run(o, tau, F, ob, 25, mask)
     with P.State(q, X, Y, Z, variant, P.FUSED) as st:
         st.set_tuning(P.capi.TUNE_SCHEME, SCHEMES[scheme])
-        st.set_tuning(P.capi.TUNE_VECTOR, 1 if X in (128, 256) else 0)
+        st.set_tuning(P.capi.TUNE_VECTOR, 1 if X in (128, 256, 384, 512) else 0)
         st.set_boundary(pb, mask)
         st.initialize(1.0, (0.01, 0, 0), init, pb)
         st.configure(tau, F, pb)
@@ -214,13 +215,14 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
 
 
 @pytest.mark.parametrize("variant", ["f64", "f32", "f32s", "f16s"])
-@pytest.mark.parametrize("X,vec", [(64, 1), (128, 1), (128, 2), (256, 1), (256, 2), (256, 4)])
+@pytest.mark.parametrize("X,vec", [(64, 1), (128, 1), (128, 2), (256, 1), (256, 2), (256, 4), (384, 1), (384, 2),
+                                   (512, 1), (512, 2), (512, 4)])
 @pytest.mark.parametrize("nt", [0, 1])
 def test_closed_box_vector_pull(orc, variant, X, vec, nt):
     """All six faces walls (the lid moving): bounce-back on x-walls happens per
     element inside the row pull kernel, whose x shifts cross lanes and (for
-    rows of several wavefronts) segments; with and without non-temporal
-    loads and stores."""
+    rows of several wavefronts) segments, every row shape the kernel is built
+    for; with and without non-temporal loads and stores."""
     print(f"\n[case] row pull {variant} X={X} vec={vec} nt={nt}", flush=True)
     q, Y, Z = 19, 6, 5
     types = [NOSLIP] * 6
