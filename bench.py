@@ -167,6 +167,7 @@ def cpu_baseline(sample_steps=150, single_core_steps=8):
         "phase_seconds": {k: round(v, 4) for k, v in zip(
             ["collide", "stream", "noslip", "periodic", "moments", "reset"], ph_all.tolist())},
         "cpu_model": cpu_model,
+        "nproc": len(os.sched_getaffinity(0)),
     }
 
 
