@@ -240,8 +240,8 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
 #define PLB_TUNE_XCD 6        /* tile mapping: 0 workgroup i takes tile i; 1 each */
                               /* XCD sweeps its own contiguous run of tiles; 2    */
                               /* each XCD takes a y-stripe of every z-plane (rows */
-                              /* % 8 == 0, else 0); -1 (default) = 2 around       */
-                              /* interior solids, else 0                          */
+                              /* % 8 == 0, else 0); -1 (default) = 2 for the cell */
+                              /* kernels around interior solids, else 0           */
 #define PLB_TUNE_INDEX64 7    /* 1: the cell kernels' 64-bit index instantiations even */
                               /* where 32-bit byte offsets fit (they run by themselves */
                               /* for buffers over 4 GiB); default 0                    */

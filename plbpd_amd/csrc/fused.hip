@@ -713,11 +713,15 @@ int launch_kind(plb_state* s, KernelKind kind) {
         a.dntx = make_fastdiv(ntx);
         a.dnty = make_fastdiv(nty);
         a.dnby = make_fastdiv(a.order ? nty / a.order : 1u);
-        // Automatic tile mapping: around interior solids the bounce-back loads
+        // Automatic tile mapping.  Around interior solids the bounce-back loads
         // read lines again that the y +- 1 and z +- 1 rows read; XCD y-stripes
-        // keep those rows in one L2 (for the cell kernel, HBM reads fell from
-        // 1.55x to 1.08x the algorithmic volume, at the same speed).
-        int xm = s->tune_xcd >= 0 ? s->tune_xcd : (s->has_solids ? 2 : 0);
+        // (2) keep those rows on one L2.  For the cell kernel that cuts HBM
+        // reads from 1.29x to 1.04x the algorithmic volume (f16s) at the same
+        // speed, so it is the cell kernel's default around solids.  The row
+        // kernel loads a row's bounce-back lines with its pulled rows, and is
+        // 1% (f32s) to 3% (f32) faster with workgroup i on tile i (0), the
+        // default everywhere else.
+        int xm = s->tune_xcd >= 0 ? s->tune_xcd : (s->has_solids && !vec ? 2 : 0);
         if (xm == 2 && nty % 8 != 0) xm = 0;
         a.xcd = (uint32_t)xm;
         a.dper = make_fastdiv(xm == 2 ? ntx * (nty / 8) : 1u);
