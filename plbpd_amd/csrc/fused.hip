@@ -660,18 +660,22 @@ int launch_kind(plb_state* s, KernelKind kind) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
     // the row pull kernel (fused_row.hpp) takes rows of 64 * vec * wpr cells
     // (kRowShapes).  Automatic choice, from measurement on MI355X (D3Q19
-    // 256^3): for 4- and 8-byte storage 1 cell per lane without interior
-    // solids, and around solids 4 (4-byte) or 2 (8-byte), else the next width
-    // with a kernel for X; for 2-byte storage the cell-per-thread kernel,
-    // which measured faster than every row width.
+    // 256^3): 1 cell per lane without interior solids, and around solids 4
+    // (4- and 2-byte storage) or 2 (8-byte), else the next width with a
+    // kernel for X; for 2-byte storage without solids, and around solids
+    // where no 4-cell shape fits, the cell-per-thread kernel (faster there
+    // than every row width measured).
     auto row_ok = [&](int v) { return v > 0 && s->pstride % (size_t)v == 0 && pull_row_rows(v, s->X) > 0; };
     int vec = 0;
     if (kind == kPull && s->tune_vector >= 0) {
         vec = row_ok(s->tune_vector) ? s->tune_vector : 0;
-    } else if (kind == kPull && s->sbytes != 2) {
-        static const int kSolid4[] = {4, 2, 1}, kSolid8[] = {2, 1}, kOpen[] = {1, 2};
-        const int* pref = s->has_solids ? (s->sbytes == 4 ? kSolid4 : kSolid8) : kOpen;
-        const int npref = s->has_solids ? (s->sbytes == 4 ? 3 : 2) : 2;
+    } else if (kind == kPull && (s->sbytes != 2 || s->has_solids)) {
+        static const int kSolid4[] = {4, 2, 1}, kSolid8[] = {2, 1}, kSolid2[] = {4}, kOpen[] = {1, 2};
+        const int* pref = kOpen;
+        int npref = 2;
+        if (s->has_solids && s->sbytes == 4) pref = kSolid4, npref = 3;
+        if (s->has_solids && s->sbytes == 8) pref = kSolid8, npref = 2;
+        if (s->has_solids && s->sbytes == 2) pref = kSolid2, npref = 1;
         for (int i = 0; i < npref && !vec; ++i)
             if (row_ok(pref[i])) vec = pref[i];
     }
