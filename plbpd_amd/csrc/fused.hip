@@ -287,20 +287,24 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
     const Cell p = cell_of(a);
     if (!p.valid) return;
     const I c = (I)p.c;
-    uint32_t m = 0;
-    bool self_solid;
-    if (NB) {
-        m = nbmask[c];
-        self_solid = m & 1u;
-    } else {
-        self_solid = flags[c] != 0;
-    }
-    if (__all(self_solid)) return;
-
     const Seams<I> sm = seams<I>(p.x, p.y, p.z, a);
     const I ps = (I)a.pstride;
     T f[Q];
-    pull_gather<Q, 0, V, NB, false>(f, A, c, ps, m, sm);
+    uint32_t m = 0;
+    bool self_solid;
+    if (NB) {
+        // around interior solids the gather addresses depend on the mask
+        m = nbmask[c];
+        self_solid = m & 1u;
+        if (__all(self_solid)) return;
+        pull_gather<Q, 0, V, NB, false>(f, A, c, ps, m, sm);
+    } else {
+        // the addresses do not depend on the flag: every load is issued
+        // before the first wait (a wavefront of wall cells loads in vain)
+        pull_gather<Q, 0, V, NB, false>(f, A, c, ps, m, sm);
+        self_solid = flags[c] != 0;
+        if (__all(self_solid)) return;
+    }
     if (COLLIDE) {
         const T rho = bgk<Q, P>(f, a, true);
         // MOV: an instantiation for boundaries with a moving wall, so that
