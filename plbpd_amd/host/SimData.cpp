@@ -29,20 +29,33 @@ SimData::~SimData() {
 void SimData::bootstrap() {
     size_t X, Y, Z;
     std::tie(X, Y, Z) = dims_;
-    rho = new Field::ScalarField<plbpd::real, 1>(Z, Y, X, 0);
-    u = new Field::VectorField<plbpd::real, 1>(Z, Y, X, 3, 0);
-    n = new Field::VectorField<plbpd::real, 1>(Z, Y, X, kdim_, 0);
-    ntmp = new Field::VectorField<plbpd::real, 1>(Z, Y, X, kdim_, 0);
-    plb_state_desc d;
-    d.q = (int32_t)kdim_;
-    d.x = (int32_t)X;
-    d.y = (int32_t)Y;
-    d.z = (int32_t)Z;
-    d.variant = cfg_.variant;
-    d.mode = cfg_.mode;
-    d.device = cfg_.device;
-    d.pop_pad = cfg_.pop_pad;
-    plbpd::check(plb_state_create(&d, &state_));
+    // A constructor that throws runs no destructor: free what was built.
+    try {
+        rho = new Field::ScalarField<plbpd::real, 1>(Z, Y, X, 0);
+        u = new Field::VectorField<plbpd::real, 1>(Z, Y, X, 3, 0);
+        n = new Field::VectorField<plbpd::real, 1>(Z, Y, X, kdim_, 0);
+        ntmp = new Field::VectorField<plbpd::real, 1>(Z, Y, X, kdim_, 0);
+        plb_state_desc d;
+        d.q = (int32_t)kdim_;
+        d.x = (int32_t)X;
+        d.y = (int32_t)Y;
+        d.z = (int32_t)Z;
+        d.variant = cfg_.variant;
+        d.mode = cfg_.mode;
+        d.device = cfg_.device;
+        d.pop_pad = cfg_.pop_pad;
+        plbpd::check(plb_state_create(&d, &state_));
+    } catch (...) {
+        delete rho;
+        delete u;
+        delete n;
+        delete ntmp;
+        rho = nullptr;
+        u = nullptr;
+        n = nullptr;
+        ntmp = nullptr;
+        throw;
+    }
 }
 
 void SimData::sync_to_host() {
