@@ -235,8 +235,9 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                               /* where the row kernel has no shape for x; -1      */
                               /* (default) = 1 without interior solids (the cell  */
                               /* kernel for 2-byte storage), around them 4 (4-    */
-                              /* and 2-byte) or 2 (8-byte), else the next width   */
-                              /* with a shape for x (2-byte: the cell kernel)     */
+                              /* and 2-byte) or 2 (8-byte, and every D3Q27        */
+                              /* state), else the next width with a shape for x   */
+                              /* (2-byte: the cell kernel)                        */
 #define PLB_TUNE_XCD 6        /* tile mapping: 0 workgroup i takes tile i; 1 each */
                               /* XCD sweeps its own contiguous run of tiles; 2    */
                               /* each XCD takes a y-stripe of every z-plane (rows */

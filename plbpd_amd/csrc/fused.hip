@@ -663,23 +663,24 @@ enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly };
 int launch_kind(plb_state* s, KernelKind kind) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
     // the row pull kernel (fused_row.hpp) takes rows of 64 * vec * wpr cells
-    // (kRowShapes).  Automatic choice, from measurement on MI355X (D3Q19
-    // 256^3): 1 cell per lane without interior solids, and around solids 4
-    // (4- and 2-byte storage) or 2 (8-byte), else the next width with a
-    // kernel for X; for 2-byte storage without solids, and around solids
-    // where no 4-cell shape fits, the cell-per-thread kernel (faster there
-    // than every row width measured).
+    // (kRowShapes).  Automatic choice, from measurement on MI355X (256^3
+    // channels): 1 cell per lane without interior solids; around solids 4
+    // (D3Q19 and D2Q9 with 4- and 2-byte storage) or 2 (8-byte storage, and
+    // D3Q27, whose 4-cell kernels need all 256 VGPRs: one wavefront per SIMD,
+    // 20% slower for f32s), else the next width with a kernel for X.  The
+    // cell-per-thread kernel for 2-byte storage without solids, and around
+    // solids where the first width does not fit (measured faster there).
     auto row_ok = [&](int v) { return v > 0 && s->pstride % (size_t)v == 0 && pull_row_rows(v, s->X) > 0; };
     int vec = 0;
     if (kind == kPull && s->tune_vector >= 0) {
         vec = row_ok(s->tune_vector) ? s->tune_vector : 0;
     } else if (kind == kPull && (s->sbytes != 2 || s->has_solids)) {
-        static const int kSolid4[] = {4, 2, 1}, kSolid8[] = {2, 1}, kSolid2[] = {4}, kOpen[] = {1, 2};
+        static const int k421[] = {4, 2, 1}, k21[] = {2, 1}, k4[] = {4}, k2[] = {2}, kOpen[] = {1, 2};
+        const bool wide = s->q == 27 || s->sbytes == 8;
         const int* pref = kOpen;
         int npref = 2;
-        if (s->has_solids && s->sbytes == 4) pref = kSolid4, npref = 3;
-        if (s->has_solids && s->sbytes == 8) pref = kSolid8, npref = 2;
-        if (s->has_solids && s->sbytes == 2) pref = kSolid2, npref = 1;
+        if (s->has_solids && s->sbytes == 2) pref = wide ? k2 : k4, npref = 1;
+        else if (s->has_solids) pref = wide ? k21 : k421, npref = wide ? 2 : 3;
         for (int i = 0; i < npref && !vec; ++i)
             if (row_ok(pref[i])) vec = pref[i];
     }
