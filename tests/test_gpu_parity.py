@@ -214,6 +214,32 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
     assert np.array_equal(rho, orho) and np.array_equal(u, ou)
 
 
+@pytest.mark.parametrize("q,X,Y,Z", [(19, 256, 6, 5), (27, 256, 4, 4), (27, 384, 3, 3)])
+@pytest.mark.parametrize("variant", ["f64", "f32", "f64a32", "f32s", "f16s"])
+def test_default_widths_around_solids(orc, q, X, Y, Z, variant):
+    """With the automatic kernel choice around interior solids (4 cells per
+    lane for D3Q19 4- and 2-byte storage, 2 for 8-byte storage and for
+    D3Q27), the fused step equals the oracle bit for bit."""
+    print(f"\n[case] default width q={q} {X}x{Y}x{Z} {variant}", flush=True)
+    ob, pb = fused_bc(q, True, lid=(0.02, 0.0, 0.0))
+    mask = (np.random.default_rng(8).random((Z, Y, X)) < 0.1).astype(np.uint8)
+    init = (P.INIT_PERTURBED, 21, 1e-3)
+    o = orc.cor_state(variant, q, X, Y, Z)
+    orc.cor_init(o, 1.0, (0.01, 0, 0), init)
+    orc.cor_run(o, 0.65, (1e-5, 0, 0), ob, 12, mask)
+    with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+        st.set_boundary(pb, mask)
+        st.initialize(1.0, (0.01, 0, 0), init, pb)
+        st.configure(0.65, (1e-5, 0, 0), pb)
+        st.step(12)
+        raw = st.download_raw()
+    fluid = orc.cor_flags(q, X, Y, Z, ob, mask) == 0
+    if variant == "f16s":
+        assert np.array_equal(raw.view(np.uint16)[fluid], o["f"].view(np.uint16)[fluid])
+    else:
+        assert np.array_equal(raw[fluid], o["f"][fluid])
+
+
 @pytest.mark.parametrize("variant", ["f64", "f32", "f32s", "f16s"])
 @pytest.mark.parametrize("X,vec", [(64, 1), (128, 1), (128, 2), (256, 1), (256, 2), (256, 4), (384, 1), (384, 2),
                                    (512, 1), (512, 2), (512, 4)])
