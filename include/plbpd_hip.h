@@ -233,11 +233,11 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                               /* 256, 384, 512 for 2; 256, 512 for 4); 0 =        */
                               /* cell-per-thread pull kernel, also                */
                               /* where the row kernel has no shape for x; -1      */
-                              /* (default) = 1 without interior solids (the cell  */
-                              /* kernel for 2-byte storage), around them 4 (4-    */
-                              /* and 2-byte) or 2 (8-byte, and every D3Q27        */
-                              /* state), else the next width with a shape for x   */
-                              /* (2-byte: the cell kernel)                        */
+                              /* (default) = for 4- and 8-byte storage 1 without  */
+                              /* interior solids, around them 4 (4-byte) or 2     */
+                              /* (8-byte, and every D3Q27 state), else the next   */
+                              /* width with a shape for x; the cell kernel for    */
+                              /* 2-byte storage                                   */
 #define PLB_TUNE_XCD 6        /* tile mapping: 0 workgroup i takes tile i; 1 each */
                               /* XCD sweeps its own contiguous run of tiles; 2    */
                               /* each XCD takes a y-stripe of every z-plane (rows */

@@ -223,6 +223,23 @@ __device__ inline void pull_gather(typename Prec<V>::T (&f)[Q], const typename P
     }
 }
 
+// Around interior solids with 2-byte storage: both candidates of every
+// population, the pulled value and the cell's own reverse slot, are loaded
+// before the neighbour mask is known (no load address waits on it); the mask
+// then selects.  On MI355X this made the 2-byte step 11% (D3Q19) to 13%
+// (D3Q27) faster; 4-byte storage was unchanged and 8-byte 2-3% slower, so
+// they keep the single load at a mask-selected address.
+template <int Q, int K, int V, class I>
+__device__ inline void pull_gather_both(typename Prec<V>::S (&fv)[Q], typename Prec<V>::S (&own)[Q],
+                                        const typename Prec<V>::S* __restrict__ A, I c, I ps,
+                                        const Seams<I>& s) {
+    if constexpr (K < Q) {
+        fv[K] = *elem(A, (I)K * ps + (c + pull_offset<Q, K>(s)));
+        if constexpr (K > 0) own[K] = *elem(A, (I)rev_k<Q>(K) * ps + c);
+        pull_gather_both<Q, K + 1, V>(fv, own, A, c, ps, s);
+    }
+}
+
 template <int Q, int V, bool NT, class I>
 __device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Prec<V>::S* __restrict__ B, I c,
                                  I ps, bool zero) {
@@ -292,8 +309,17 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
     T f[Q];
     uint32_t m = 0;
     bool self_solid;
-    if (NB) {
-        // around interior solids the gather addresses depend on the mask
+    if (NB && sizeof(typename P::S) == 2) {
+        typename P::S fv[Q], own[Q];
+        pull_gather_both<Q, 0, V>(fv, own, A, c, ps, sm);
+        m = nbmask[c];
+        self_solid = m & 1u;
+        if (__all(self_solid)) return;
+#pragma unroll
+        for (int k = 0; k < Q; ++k)
+            f[k] = P::load_raw(k > 0 && ((m >> rev_k<Q>(k)) & 1u) ? own[k] : fv[k]);
+    } else if (NB) {
+        // the gather addresses depend on the mask
         m = nbmask[c];
         self_solid = m & 1u;
         if (__all(self_solid)) return;
@@ -664,23 +690,22 @@ int launch_kind(plb_state* s, KernelKind kind) {
     if (!s->bc_set) return fail(PLB_ERR_LOGIC, "fused step before the boundary was set");
     // the row pull kernel (fused_row.hpp) takes rows of 64 * vec * wpr cells
     // (kRowShapes).  Automatic choice, from measurement on MI355X (256^3
-    // channels): 1 cell per lane without interior solids; around solids 4
-    // (D3Q19 and D2Q9 with 4- and 2-byte storage) or 2 (8-byte storage, and
-    // D3Q27, whose 4-cell kernels need all 256 VGPRs: one wavefront per SIMD,
-    // 20% slower for f32s), else the next width with a kernel for X.  The
-    // cell-per-thread kernel for 2-byte storage without solids, and around
-    // solids where the first width does not fit (measured faster there).
+    // channels): for 4- and 8-byte storage 1 cell per lane without interior
+    // solids; around solids 4 (D3Q19 and D2Q9, 4-byte storage) or 2 (8-byte
+    // storage, and D3Q27, whose 4-cell kernels need all 256 VGPRs: one
+    // wavefront per SIMD, 20% slower for f32s), else the next width with a
+    // kernel for X.  2-byte storage takes the cell-per-thread kernel, which
+    // measured as fast as the best row width for D3Q19 and faster for D3Q27.
     auto row_ok = [&](int v) { return v > 0 && s->pstride % (size_t)v == 0 && pull_row_rows(v, s->X) > 0; };
     int vec = 0;
     if (kind == kPull && s->tune_vector >= 0) {
         vec = row_ok(s->tune_vector) ? s->tune_vector : 0;
-    } else if (kind == kPull && (s->sbytes != 2 || s->has_solids)) {
-        static const int k421[] = {4, 2, 1}, k21[] = {2, 1}, k4[] = {4}, k2[] = {2}, kOpen[] = {1, 2};
+    } else if (kind == kPull && s->sbytes != 2) {
+        static const int k421[] = {4, 2, 1}, k21[] = {2, 1}, kOpen[] = {1, 2};
         const bool wide = s->q == 27 || s->sbytes == 8;
         const int* pref = kOpen;
         int npref = 2;
-        if (s->has_solids && s->sbytes == 2) pref = wide ? k2 : k4, npref = 1;
-        else if (s->has_solids) pref = wide ? k21 : k421, npref = wide ? 2 : 3;
+        if (s->has_solids) pref = wide ? k21 : k421, npref = wide ? 2 : 3;
         for (int i = 0; i < npref && !vec; ++i)
             if (row_ok(pref[i])) vec = pref[i];
     }
