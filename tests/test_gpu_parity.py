@@ -217,9 +217,10 @@ def test_fused_bitexact(orc, q, X, Y, Z, walls, solids, variant, scheme):
 @pytest.mark.parametrize("q,X,Y,Z", [(19, 256, 6, 5), (27, 256, 4, 4), (27, 384, 3, 3)])
 @pytest.mark.parametrize("variant", ["f64", "f32", "f64a32", "f32s", "f16s"])
 def test_default_widths_around_solids(orc, q, X, Y, Z, variant):
-    """With the automatic kernel choice around interior solids (4 cells per
-    lane for D3Q19 4- and 2-byte storage, 2 for 8-byte storage and for
-    D3Q27), the fused step equals the oracle bit for bit."""
+    """With the automatic kernel choice around interior solids (row kernel
+    with 4 cells per lane for D3Q19 4-byte storage, 2 for 8-byte storage and
+    for D3Q27; the cell kernel, both candidates loaded, for 2-byte storage),
+    the fused step equals the oracle bit for bit."""
     print(f"\n[case] default width q={q} {X}x{Y}x{Z} {variant}", flush=True)
     ob, pb = fused_bc(q, True, lid=(0.02, 0.0, 0.0))
     mask = (np.random.default_rng(8).random((Z, Y, X)) < 0.1).astype(np.uint8)
