@@ -206,11 +206,12 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->graph_parity = 0;
     s->timers_on = false;
     s->tune_tx = 0;
-    // defaults measured fastest on MI355X for D3Q19 256^3 (FP64, FP32, FP16
-    // storage, with and without interior solids): non-temporal stores, linear
-    // row order, the row pull kernel where the row length allows it (width by
-    // storage size and solids), workgroup i on tile i without interior
-    // solids and XCD y-stripes around them
+    // defaults measured fastest on MI355X for D3Q19 and D3Q27 at 256^3 and
+    // 384^3 (FP64, FP32, FP16 storage, with and without interior solids):
+    // non-temporal stores, linear row order, the row pull kernel for 4- and
+    // 8-byte storage where the row length allows it (width by storage size,
+    // lattice and solids) and the cell kernel for 2-byte storage; workgroup
+    // i on tile i, except XCD y-stripes for the cell kernel around solids
     s->tune_nt_load = 0;
     s->tune_nt_store = 1;
     s->tune_scheme = PLB_SCHEME_AUTO;
