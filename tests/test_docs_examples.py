@@ -28,3 +28,21 @@ def test_integration_c_example_compiles_and_links(tmp_path):
                         os.path.join(ROOT, "include"), "-L", os.path.dirname(LIB), "-lplbpd_hip",
                         "-Wl,--no-undefined", "-o", str(out)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def python_example():
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    section = text[text.index("## 4. Python (ctypes) binding"):]
+    return re.search(r"```python\n(.*?)```", section, re.S).group(1)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="needs the built library")
+def test_integration_ctypes_stub_runs(monkeypatch):
+    """The ctypes stub runs as written: it creates and destroys a state on a
+    GPU, and without one fails through its own check() with the library's
+    no-device message."""
+    monkeypatch.chdir(ROOT)
+    try:
+        exec(compile(python_example(), "INTEGRATION.md", "exec"), {})
+    except RuntimeError as e:
+        assert "no HIP device" in str(e)
