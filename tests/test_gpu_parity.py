@@ -166,6 +166,9 @@ FUSED_CASES = [  # q, X, Y, Z, walls?, solids?
     (27, 128, 4, 5, True, True),
     (27, 384, 3, 4, True, False),
     (9, 256, 1, 8, True, False),
+    # tiny and ragged: a 1 x 1 column that wraps onto itself, a walled 2 x 3 x 3 box
+    (19, 1, 1, 4, False, False),
+    (27, 2, 3, 3, True, False),
 ]
 SCHEMES = {"push": 1, "pull": 2}
 
