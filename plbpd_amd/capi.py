@@ -253,18 +253,26 @@ class State:
         return b.value
 
     # --- data ---
+    def _field_shape(self, field, layout):
+        vlen = {FIELD_RHO: 1, FIELD_U: 3, FIELD_N: self.q, FIELD_NTMP: self.q}[field]
+        return self.shape(layout) + ((vlen,) if vlen > 1 else ())
+
     def upload(self, field, arr, layout=LAYOUT_REFERENCE):
         a = np.ascontiguousarray(arr)
-        dt = DTYPE_F64 if a.dtype == np.float64 else DTYPE_F32
         if a.dtype not in (np.float64, np.float32):
             raise TypeError("float32 or float64 arrays only")
+        shp = self._field_shape(field, layout)
+        if a.size != int(np.prod(shp)):
+            raise ValueError(f"array of {a.size} values for a field of shape {shp}")
+        dt = DTYPE_F64 if a.dtype == np.float64 else DTYPE_F32
         check(lib().plb_upload(self.h, field, layout, dt, _p(a)))
 
     def download(self, field, layout=LAYOUT_REFERENCE, dtype=np.float64, out=None):
-        vlen = {FIELD_RHO: 1, FIELD_U: 3, FIELD_N: self.q, FIELD_NTMP: self.q}[field]
-        shp = self.shape(layout) + ((vlen,) if vlen > 1 else ())
+        shp = self._field_shape(field, layout)
         if out is None:
             out = np.zeros(shp, dtype)
+        if out.dtype not in (np.float64, np.float32) or out.size != int(np.prod(shp)):
+            raise ValueError(f"out must be a float32 or float64 array of shape {shp}")
         dt = DTYPE_F64 if out.dtype == np.float64 else DTYPE_F32
         check(lib().plb_download(self.h, field, layout, dt, _p(out)))
         return out
@@ -282,6 +290,8 @@ class State:
 
     def set_boundary(self, bc: BoundaryC, mask=None):
         m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        if m is not None and m.size != self.X * self.Y * self.Z:
+            raise ValueError(f"mask of {m.size} bytes for {self.Z} x {self.Y} x {self.X} cells")
         check(lib().plb_set_boundary(self.h, C.byref(bc), None if m is None else _p(m)))
 
     # --- reference operator calls (PLBPD.cpp:24-29) ---

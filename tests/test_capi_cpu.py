@@ -48,3 +48,30 @@ def test_no_device_here_is_reported():
             P.State(19, 4, 4, 4)
         return
     pytest.skip("a GPU is visible")
+
+
+def _hostless_state(q=19, X=4, Y=3, Z=2):
+    """A State object without a device state: enough for the host-side
+    argument checks, which run before any library call."""
+    from plbpd_amd import capi
+    st = object.__new__(capi.State)
+    st.q, st.X, st.Y, st.Z, st.h = q, X, Y, Z, None
+    st.variant, st.mode = capi.F64, capi.FUSED
+    return st
+
+
+def test_wrapper_rejects_wrong_sized_arrays():
+    """Arrays handed to the library as raw pointers are checked for size
+    first, so a wrong shape raises instead of reading or writing past the
+    host buffer."""
+    import numpy as np
+    from plbpd_amd import capi
+    st = _hostless_state()
+    with pytest.raises(ValueError):
+        st.set_boundary(capi.channel(), np.zeros((2, 3, 3), np.uint8))
+    with pytest.raises(ValueError):
+        st.upload(capi.FIELD_RHO, np.zeros((4, 5, 5)))  # reference layout is (Z+2, Y+2, X+2)
+    with pytest.raises(ValueError):
+        st.download(capi.FIELD_U, capi.LAYOUT_COMPACT, out=np.zeros((2, 3, 4)))  # u has 3 components
+    with pytest.raises(TypeError):
+        st.upload(capi.FIELD_RHO, np.zeros((4, 5, 6), np.int32))
