@@ -350,8 +350,16 @@ int ensure_scratch(plb_state* s, size_t bytes) {
     return PLB_OK;
 }
 
+// The per-cell stages (collide, push, moments) launch one block per x-row
+// piece of up to 256 interior cells (whole wavefronts), the row's y and z
+// taken from the block index.
+unsigned interior_block(const plb_state* s) {
+    const unsigned w = (unsigned)((s->X + 63) / 64) * 64u;
+    return w < 256u ? w : 256u;
+}
 dim3 interior_grid(const plb_state* s) {
-    return dim3((unsigned)((s->X + 63) / 64), (unsigned)s->Y, (unsigned)s->Z);
+    const unsigned b = interior_block(s);
+    return dim3((unsigned)((s->X + b - 1) / b), (unsigned)s->Y, (unsigned)s->Z);
 }
 
 }  // namespace
@@ -388,7 +396,7 @@ int lit_collide(plb_state* s, double tau, const double F[3]) {
             om = (R)(1.0 / tau);
         }
         omm = (R)1 - om;
-        hipLaunchKernelGGL((k_collide<Q, R>), interior_grid(s), dim3(64), 0, s->stream, g,
+        hipLaunchKernelGGL((k_collide<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
                            (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, tr, om, omm,
                            (R)F[0], (R)F[1], (R)F[2]);
         PLB_HIP_TRY(hipGetLastError());
@@ -401,7 +409,7 @@ int lit_push(plb_state* s) {
     const int rc = with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
         using R = decltype(rr);
-        hipLaunchKernelGGL((k_push<Q, R>), interior_grid(s), dim3(64), 0, s->stream, g,
+        hipLaunchKernelGGL((k_push<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
                            (const R*)s->buf[s->cur], (R*)s->buf[s->cur ^ 1]);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
@@ -463,7 +471,7 @@ int lit_moments(plb_state* s) {
     return with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
         using R = decltype(rr);
-        hipLaunchKernelGGL((k_moments<Q, R>), interior_grid(s), dim3(64), 0, s->stream, g,
+        hipLaunchKernelGGL((k_moments<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
                            (const R*)s->buf[s->cur], (R*)s->rho, (R*)s->u);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
