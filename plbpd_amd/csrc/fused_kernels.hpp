@@ -108,7 +108,7 @@ __device__ inline typename P::T bgk(typename P::T (&f)[Q], const FusedArgs<typen
 
 // Row pull step (fused_row.hpp): rows of X == 64 * vec * wpr cells, vec cells
 // per lane, wpr wavefronts per row.  The shapes it is built for (the launcher
-// in fused_row.hpp has one branch per entry):
+// in fused_row.hpp is generated from this table):
 struct RowShape {
     int vec, wpr;
 };

@@ -273,22 +273,29 @@ void launch_one(const FusedArgs<typename Prec<V>::T>& a, bool nts, bool nb, cons
 
 }  // namespace
 
-// The row pull step for (vec, X), one branch per entry of kRowShapes.
+// The row kernel of shape kRowShapes[I] if it matches (vec, X), else the next
+// shape: the dispatch is generated from the table, so the two cannot differ.
+template <int Q, int V, size_t I = 0>
+bool launch_row_shape(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts, bool nb,
+                      const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
+                      const uint32_t* nbmask, unsigned nblocks, hipStream_t stream) {
+    if constexpr (I < sizeof(kRowShapes) / sizeof(kRowShapes[0])) {
+        constexpr RowShape r = kRowShapes[I];
+        if (vec == r.vec && a.X == 64u * r.vec * r.wpr) {
+            launch_one<Q, V, r.vec, r.wpr>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);
+            return true;
+        }
+        return launch_row_shape<Q, V, I + 1>(a, vec, nts, nb, A, B, flags, nbmask, nblocks, stream);
+    } else {
+        return false;
+    }
+}
+
 template <int Q, int V>
 bool launch_pull_row(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts, bool nb,
                      const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
                      const uint32_t* nbmask, unsigned nblocks, hipStream_t stream) {
-    const uint32_t X = a.X;
-#define PLB_SHAPE(VECv, WPRv)                                                                      \
-    if (vec == VECv && X == 64u * VECv * WPRv) {                                                   \
-        launch_one<Q, V, VECv, WPRv>(a, nts, nb, A, B, flags, nbmask, nblocks, stream);            \
-        return true;                                                                               \
-    }
-    PLB_SHAPE(1, 1) PLB_SHAPE(1, 2) PLB_SHAPE(1, 4) PLB_SHAPE(1, 6) PLB_SHAPE(1, 8)
-    PLB_SHAPE(2, 1) PLB_SHAPE(2, 2) PLB_SHAPE(2, 3) PLB_SHAPE(2, 4)
-    PLB_SHAPE(4, 1) PLB_SHAPE(4, 2)
-#undef PLB_SHAPE
-    return false;
+    return launch_row_shape<Q, V>(a, vec, nts, nb, A, B, flags, nbmask, nblocks, stream);
 }
 
 #define PLB_ROW_INST(Q, V)                                                                                  \
