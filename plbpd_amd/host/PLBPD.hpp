@@ -22,7 +22,10 @@ class PLBPD final {
     void initialize(SimData& simdata) const;
     void march_in_time(size_t num_timesteps, SimData& simdata) const;
     void print_times() const;
-    // GPU time per stage of the last state stepped (HIP events), ms.
+    // GPU time per stage on this state (HIP events), in seconds, accumulated
+    // while its timers are on (plbpd::DeviceConfig::device_timers); the
+    // reference-interface timers of print_times measure the host calls, which
+    // return before the GPU work ends.
     void print_device_times(SimData& simdata) const;
 
  private:

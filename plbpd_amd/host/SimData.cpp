@@ -45,6 +45,7 @@ void SimData::bootstrap() {
         d.device = cfg_.device;
         d.pop_pad = cfg_.pop_pad;
         plbpd::check(plb_state_create(&d, &state_));
+        if (cfg_.device_timers) plbpd::check(plb_enable_timers(state_, 1));
     } catch (...) {
         delete rho;
         delete u;

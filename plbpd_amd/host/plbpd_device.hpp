@@ -28,6 +28,11 @@ struct DeviceConfig {
     int mode = PLB_MODE_LITERAL;    // PLB_MODE_LITERAL or PLB_MODE_FUSED
     int device = 0;
     int pop_pad = -1;
+    // Time every stage on the GPU with HIP events (plb_enable_timers): the
+    // device-side counterpart of the reference's per-class timers, which
+    // PLBPD::print_device_times reports.  Off by default (an event pair per
+    // call).
+    bool device_timers = false;
 };
 
 void set_default_device_config(const DeviceConfig& cfg);

@@ -2,7 +2,7 @@
 // scenario, storage variant, step semantics and step count as options.
 //   plbpd_run [Couette2D|Couette3D|Poiseuille2D|Poiseuille3D|<scenario file>]
 //             [--fused] [--variant f64|f32|f64a32|f32s|f16s] [--steps N]
-//             [--out FILE]
+//             [--out FILE] [--device-times]
 // A scenario file (plbpd_scenario.hpp) may set the variant, the mode, the
 // initial state and interior solids; command-line options take precedence.
 #include <cstdlib>
@@ -19,13 +19,14 @@
 int main(int argc, char** argv) {
     std::string flow_name = "Couette3D", out = "final_state.h5", variant = "f64";
     long steps = -1;
-    bool fused = false, variant_given = false;
+    bool fused = false, variant_given = false, device_times = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "--fused") fused = true;
         else if (a == "--variant" && i + 1 < argc) variant = argv[++i], variant_given = true;
         else if (a == "--steps" && i + 1 < argc) steps = std::atol(argv[++i]);
         else if (a == "--out" && i + 1 < argc) out = argv[++i];
+        else if (a == "--device-times") device_times = true;
         else if (a[0] != '-') flow_name = a;
         else {
             std::cerr << "unknown option " << a << "\n";
@@ -34,6 +35,7 @@ int main(int argc, char** argv) {
     }
     plbpd::DeviceConfig cfg;
     cfg.mode = fused ? PLB_MODE_FUSED : PLB_MODE_LITERAL;
+    cfg.device_timers = device_times;
     if (variant == "f64") cfg.variant = PLB_F64;
     else if (variant == "f32") cfg.variant = PLB_F32;
     else if (variant == "f64a32") cfg.variant = PLB_F64_A32;
@@ -62,7 +64,7 @@ int main(int argc, char** argv) {
         problem->march_in_time(steps >= 0 ? (size_t)steps : flow->get_num_timesteps(), simdata);
         simdata.write_state(out);
         problem->print_times();
-        problem->print_device_times(simdata);
+        if (device_times) problem->print_device_times(simdata);
         std::cout << "wrote " << out << (plbpd::state_files_are_hdf5() ? " (HDF5)" : " (raw)") << "\n";
     } catch (const std::exception& e) {
         std::cerr << "plbpd_run: " << e.what() << "\n";

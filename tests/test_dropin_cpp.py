@@ -53,3 +53,17 @@ def test_runner_scenario_file(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Time taken by timeloop" in r.stdout
     assert out.exists() and out.stat().st_size > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args,stage", [(["Couette3D", "--steps", "30"], "device collide:"),
+                                        (["Couette3D", "--steps", "30", "--fused"], "device fused:")])
+def test_runner_device_times(tmp_path, args, stage):
+    """With --device-times the runner switches the state's HIP-event timers on
+    (DeviceConfig::device_timers) and prints the GPU time of each stage."""
+    out = tmp_path / "state.h5"
+    r = subprocess.run([RUN, *args, "--device-times", "--out", str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [l for l in r.stdout.splitlines() if l.startswith(stage)]
+    assert line and float(line[0].split(":")[1].strip().rstrip("s")) > 0, r.stdout
