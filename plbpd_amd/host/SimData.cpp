@@ -18,6 +18,16 @@ SimData::SimData(const std::tuple<size_t, size_t, size_t> domain_dimensions, con
     bootstrap();
 }
 
+SimData::SimData(SimData&& other) noexcept
+    : rho(other.rho), u(other.u), n(other.n), ntmp(other.ntmp), dims_(other.dims_),
+      kdim_(other.kdim_), cfg_(other.cfg_), state_(other.state_) {
+    other.rho = nullptr;
+    other.u = nullptr;
+    other.n = nullptr;
+    other.ntmp = nullptr;
+    other.state_ = nullptr;
+}
+
 SimData::~SimData() {
     delete rho;
     delete u;

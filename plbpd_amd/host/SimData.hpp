@@ -25,6 +25,12 @@ class SimData final {
     SimData(const std::tuple<size_t, size_t, size_t> domain_dimensions, const size_t num_directions,
             const plbpd::DeviceConfig& cfg);
     SimData(const SimData&) = delete;
+    // Takes over the device state and the host mirror; the source keeps
+    // nothing and may only be destroyed.  The reference's main.cpp:22-23
+    // writes `auto simdata = SimData(...)`, which C++14 (the reference's
+    // standard, src/CMakeLists.txt:6) accepts only with a copy or move
+    // constructor to elide.
+    SimData(SimData&& other) noexcept;
     SimData& operator=(SimData&) = delete;
     ~SimData();
 

@@ -5,6 +5,7 @@
 //   test_dropin gpu   PLBPD::march_in_time through the drop-in classes on the
 //                     GPU, compared bit for bit with the CPU oracle, in literal
 //                     and fused mode; state files; error types of device calls
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -429,6 +430,49 @@ void test_state_file_resume() {
     std::remove(path.c_str());
 }
 
+// TestSimData.cpp:8-15: a SimData built from a (5, 4, 3) tuple with 19
+// directions writes its bootstrapped state.  The reference's fields start at
+// zero (SimData.cpp:53-56); so must the device state read back here.
+void test_simdata_bootstrap_write() {
+    const size_t zdim = 5, ydim = 4, xdim = 3, kdim = 19;
+    auto simdata = std::make_unique<SimData>(std::make_tuple(zdim, ydim, xdim), kdim);
+    const std::string path = "/tmp/plbpd_dropin_bootstrap.h5";
+    simdata->write_state(path);
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    EXPECT(f != nullptr);
+    if (f) std::fclose(f);
+    const auto all_zero = [](const plbpd::real* p, size_t n) {
+        return std::all_of(p, p + n, [](plbpd::real v) { return v == 0; });
+    };
+    EXPECT(all_zero(simdata->rho->data(), simdata->rho->size()));
+    EXPECT(all_zero(simdata->u->data(), simdata->u->size()));
+    EXPECT(all_zero(simdata->n->data(), simdata->n->size()));
+    EXPECT(all_zero(simdata->ntmp->data(), simdata->ntmp->size()));
+    EXPECT(simdata->rho->size() == (zdim + 2) * (ydim + 2) * (xdim + 2));
+    std::remove(path.c_str());
+}
+
+// A SimData moved mid-run carries its device state along: 10 steps, a move,
+// 10 more steps equal 20 steps on one object, bit for bit.
+void test_move_keeps_state() {
+    Setup s = couette(10, 6, 8, 0.75, {1e-5, 0, 0}, 0.02f);
+    plbpd::DeviceConfig cfg;
+    PLBPD p(s.model.get(), s.domain.get(), s.boundary.get());
+    auto a = SimData(s.domain->get_dimensions(), 19, cfg);
+    SimData ref(s.domain->get_dimensions(), 19, cfg);
+    p.initialize(a);
+    p.initialize(ref);
+    p.march_in_time(10, a);
+    SimData b(std::move(a));
+    EXPECT(a.device() == nullptr && a.n == nullptr);
+    p.march_in_time(10, b);
+    p.march_in_time(20, ref);
+    b.sync_to_host();
+    ref.sync_to_host();
+    EXPECT(std::memcmp(b.n->data(), ref.n->data(), b.n->size() * sizeof(double)) == 0);
+    EXPECT(std::memcmp(b.u->data(), ref.u->data(), b.u->size() * sizeof(double)) == 0);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -448,6 +492,8 @@ int main(int argc, char** argv) {
         test_scenario_march_matches_oracle();
         test_device_errors();
         test_state_file_resume();
+        test_simdata_bootstrap_write();
+        test_move_keeps_state();
     }
     std::printf("%s: %d failure(s)\n", what.c_str(), g_fail);
     return g_fail ? 1 : 0;

@@ -26,6 +26,30 @@ def test_dropin_march_in_time_matches_oracle():
     assert r.returncode == 0, r.stdout + r.stderr
 
 
+MAIN14 = os.path.join(ROOT, "tests", "cpp", "_build", "main_cxx14")
+
+
+def test_reference_main_compiles_as_cxx14():
+    """The reference's main() call sequence compiles as C++14 (the reference's
+    standard, src/CMakeLists.txt:6) against the drop-in headers."""
+    src = os.path.join(ROOT, "tests", "cpp", "main_cxx14.cpp")
+    r = subprocess.run(["g++", "-std=c++14", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", src,
+                        "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "plbpd_amd", "host")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_reference_main_runs(tmp_path):
+    """The same program on the GPU: Couette3D (200x50x50 D3Q19, 200 steps,
+    literal FP64), writing the initial and the final state."""
+    r = subprocess.run([MAIN14], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Time taken by timeloop" in r.stdout
+    init, final = (tmp_path / "init_state.h5").read_bytes(), (tmp_path / "final_state.h5").read_bytes()
+    assert len(init) == len(final) > 0 and init != final
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("args", [["Couette3D", "--steps", "20"],
                                   ["Poiseuille2D", "--steps", "200", "--fused", "--variant", "f32s"]])
