@@ -32,6 +32,10 @@ class Domain final {
     std::tuple<size_t, size_t, size_t> get_dimensions() const;
     float get_fluid_viscosity() const;
     float get_fluid_density() const;
+    // Declared by the reference (Domain.hpp:36) and defined nowhere: a
+    // Domain holds no solid density (Boundary does), so a call fails to link
+    // there and here alike.
+    float get_solid_density() const;
     const std::array<float, 3>& get_init_flow_velocity() const;
     const std::array<float, 3>& get_external_force() const;
     void initialize(SimData& simdata) const;

@@ -39,6 +39,18 @@ def test_reference_main_compiles_as_cxx14():
     assert r.returncode == 0, r.stderr
 
 
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_reference_interface_signatures(real):
+    """Every public member of the reference's headers exists in the drop-in
+    with the same type (tests/cpp/api_cxx14.cpp, static assertions), under
+    C++14, for double fields and for the reference's float fields."""
+    src = os.path.join(ROOT, "tests", "cpp", "api_cxx14.cpp")
+    r = subprocess.run(["g++", "-std=c++14", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", f"-DPLBPD_REAL={real}",
+                        src, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "plbpd_amd", "host")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
 @pytest.mark.gpu
 def test_reference_main_runs(tmp_path):
     """The same program on the GPU: Couette3D (200x50x50 D3Q19, 200 steps,
