@@ -134,7 +134,11 @@ int plb_lattice_tables(int q, int32_t* c, double* w, uint32_t* rev);
 /* ---- state: SimData (SimData.hpp:9-31) on the device ---------------------- */
 /* SimData(domain_dimensions, num_directions) (SimData.cpp:22-30): allocates
  * the device population buffers (A/B, the n/ntmp pair of SimData.hpp:20-21),
- * rho and u.  A zero extent -> PLB_ERR_INVALID_ARGUMENT (Field.hpp:28-31). */
+ * rho and u.  A zero extent -> PLB_ERR_INVALID_ARGUMENT (Field.hpp:28-31).
+ * Sizes no device holds are refused by size alone (PLB_ERR_INVALID_ARGUMENT,
+ * "domain too large"), before any device call: fused states take fewer than
+ * 2^32 cells, literal states fewer than 2^40 halo'd cells, with halo'd x and
+ * y edges below 2^31. */
 int plb_state_create(const plb_state_desc* desc, plb_state** out);
 int plb_state_destroy(plb_state* s);
 /* Bytes of device memory the state holds.  No reference counterpart. */
@@ -260,7 +264,9 @@ typedef struct {
 } plb_domain_desc;
 
 /* ndom independent domains sharing q, variant and boundary kinds, packed in
- * one pool of device memory.  bc applies to every domain (fused semantics). */
+ * one pool of device memory.  bc applies to every domain (fused semantics).
+ * Each domain has fewer than 2^31 cells (else PLB_ERR_INVALID_ARGUMENT,
+ * "domain too large for a batch"). */
 int plb_batch_create(int q, int variant, int ndom, const plb_domain_desc* doms,
                      const plb_boundary* bc, int device, plb_batch** out);
 int plb_batch_destroy(plb_batch* b);

@@ -559,14 +559,18 @@ int plb_batch_create(int q, int variant, int ndom, const plb_domain_desc* doms, 
     for (int d = 0; d < ndom; ++d) {
         const plb_domain_desc& s = doms[d];
         if (s.x <= 0 || s.y <= 0 || s.z <= 0) return fail(PLB_ERR_INVALID_ARGUMENT, "argument is zero");
-        const uint64_t n = (uint64_t)s.x * s.y * s.z;
-        if (n >= (1ull << 31)) return fail(PLB_ERR_INVALID_ARGUMENT, "domain too large for a batch");
+        // each factor is below 2^31, so neither product can wrap
+        const uint64_t xy = (uint64_t)s.x * (uint64_t)s.y;
+        if (xy >= (1ull << 31) || xy * (uint64_t)s.z >= (1ull << 31))
+            return fail(PLB_ERR_INVALID_ARGUMENT, "domain too large for a batch");
+        const uint64_t n = xy * (uint64_t)s.z;
         if (!(s.tau > 0.5)) return fail(PLB_ERR_INVALID_ARGUMENT, "tau must exceed 1/2");
         int rc = check_fused_bc(q, s.x, s.y, s.z, bc);
         if (rc) return rc;
         off[d] = pool;
         ps[d] = (n + 63) / 64 * 64;
-        pool += ps[d] * q;
+        pool += ps[d] * q;  // below 2^31 * 27 per domain
+        if (pool >= (1ull << 56)) return fail(PLB_ERR_INVALID_ARGUMENT, "batch too large");
         total += n;
         // wall layers of no-slip faces are not fluid (D2Q9 never moves in y)
         const uint64_t fx = s.x - (bc->type[0] == kNoslip ? 2 : 0);

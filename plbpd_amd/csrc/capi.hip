@@ -169,6 +169,18 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     if (d->x <= 0 || d->y <= 0 || d->z <= 0) return fail(PLB_ERR_INVALID_ARGUMENT, "argument is zero");
     if (d->mode != PLB_MODE_LITERAL && d->mode != PLB_MODE_FUSED)
         return fail(PLB_ERR_INVALID_ARGUMENT, "unknown mode");
+    // Cell counts that no device holds are refused before any size product
+    // can wrap: the fused kernels number cells, rows and tiles in 32 bits, and
+    // the literal kernels keep halo'd row lengths in int.
+    {
+        const uint64_t h = d->mode == PLB_MODE_LITERAL ? 2 : 0;
+        const uint64_t ex = (uint64_t)d->x + h, ey = (uint64_t)d->y + h, ez = (uint64_t)d->z + h;
+        const uint64_t lim = d->mode == PLB_MODE_LITERAL ? (1ull << 40) : (1ull << 32);
+        // ex, ey < 2^31 keep ex * ey from wrapping; the division keeps the
+        // product with ez from wrapping (ex * ey * ez <= lim - 1 exactly then)
+        if (ex >= (1ull << 31) || ey >= (1ull << 31) || ex * ey > (lim - 1) / ez)
+            return fail(PLB_ERR_INVALID_ARGUMENT, "domain too large");
+    }
     if (storage_bytes(d->variant) == 0) return fail(PLB_ERR_INVALID_ARGUMENT, "unknown storage variant");
     if (d->mode == PLB_MODE_LITERAL && d->variant != kF64 && d->variant != kF32)
         return fail(PLB_ERR_INVALID_ARGUMENT, "literal mode supports PLB_F64 and PLB_F32 only");

@@ -50,6 +50,38 @@ def test_no_device_here_is_reported():
     pytest.skip("a GPU is visible")
 
 
+@pytest.mark.parametrize("mode", [capi.LITERAL, capi.FUSED])
+@pytest.mark.parametrize("dims", [(1 << 22, 1 << 22, 1 << 22),        # X*Y*Z wraps to 0 in 64 bits
+                                  ((1 << 31) - 1,) * 3,
+                                  (1 << 16, 1 << 16, 1 << 16)])        # 2^48 cells
+def test_oversized_domains_are_refused_before_any_device_call(mode, dims):
+    """Cell counts that no device holds are refused by their size alone,
+    before a product of the edges can wrap and size a buffer too small for
+    the kernels' indices.  The check needs no GPU."""
+    with pytest.raises(P.InvalidArgument, match="too large"):
+        P.State(19, *dims, variant=capi.F64, mode=mode)
+
+
+def test_literal_row_length_limit():
+    """Literal states keep the halo'd row length X + 2 in an int."""
+    with pytest.raises(P.InvalidArgument, match="too large"):
+        P.State(19, (1 << 31) - 2, 1, 1, variant=capi.F64, mode=capi.LITERAL)
+
+
+def test_fused_cell_limit_is_2_to_the_32():
+    """Fused states number cells in 32 bits: 2^32 cells are refused by size."""
+    with pytest.raises(P.InvalidArgument, match="too large"):
+        P.State(19, 1 << 16, 1 << 8, 1 << 8, variant=capi.F32S, mode=capi.FUSED)
+
+
+def test_oversized_batch_domain_is_refused():
+    """A batch domain whose edges multiply past 2^64 (to 0) is refused, not
+    taken for an empty domain."""
+    dom = {"x": 1 << 22, "y": 1 << 22, "z": 1 << 22, "tau": 0.8, "force": (0, 0, 0)}
+    with pytest.raises(P.InvalidArgument, match="too large"):
+        capi.Batch(19, capi.F32S, [dom], capi.channel())
+
+
 def _hostless_state(q=19, X=4, Y=3, Z=2):
     """A State object without a device state: enough for the host-side
     argument checks, which run before any library call."""
