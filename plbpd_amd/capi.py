@@ -210,6 +210,14 @@ def lattice_tables(q: int):
     return c, w, rev
 
 
+def _check_edges(*edges):
+    """The descriptors hold int32 edges, and ctypes would truncate a wider
+    one silently; the library refuses zero and oversized extents itself."""
+    for e in edges:
+        if int(e) > 0x7FFFFFFF:
+            raise InvalidArgument("domain too large")
+
+
 class State:
     """A device lattice (the device side of SimData) with one method per
     reference operator call."""
@@ -217,6 +225,7 @@ class State:
     def __init__(self, q, X, Y, Z, variant=F64, mode=LITERAL, device=0, pop_pad=-1):
         if isinstance(variant, str):
             variant = VARIANT_NAMES[variant]
+        _check_edges(X, Y, Z)
         self.q, self.X, self.Y, self.Z = q, X, Y, Z
         self.variant, self.mode = variant, mode
         d = StateDesc(q, X, Y, Z, variant, mode, device, pop_pad)
@@ -367,6 +376,7 @@ class Batch:
             variant = VARIANT_NAMES[variant]
         arr = (DomainDesc * len(domains))()
         for i, dm in enumerate(domains):
+            _check_edges(dm["x"], dm["y"], dm["z"])
             arr[i].x, arr[i].y, arr[i].z = dm["x"], dm["y"], dm["z"]
             arr[i].tau = float(dm["tau"])
             for j in range(3):

@@ -910,7 +910,7 @@ int fused_transfer(plb_state* s, int field, int layout, int dtype, void* host, b
     if (upload && field != PLB_FIELD_N)
         return fail(PLB_ERR_LOGIC, "rho and u are derived from the populations in fused mode");
     const int vlen = field == PLB_FIELD_N ? s->q : (field == PLB_FIELD_RHO ? 1 : 3);
-    const uint64_t hcells = ref ? (uint64_t)(s->X + 2) * (s->Y + 2) * (s->Z + 2) : n;
+    const uint64_t hcells = ref ? ((uint64_t)s->X + 2) * ((uint64_t)s->Y + 2) * ((uint64_t)s->Z + 2) : n;
     const size_t esz = dtype == PLB_DTYPE_F64 ? 8 : 4;
     const size_t hbytes = hcells * vlen * esz;
     if (field != PLB_FIELD_N && !s->moments_valid) {

@@ -1,5 +1,7 @@
 #include "SimData.hpp"
 
+#include <cstdint>
+#include <stdexcept>
 #include <type_traits>
 
 #include "plbpd_state.hpp"
@@ -39,6 +41,9 @@ SimData::~SimData() {
 void SimData::bootstrap() {
     size_t X, Y, Z;
     std::tie(X, Y, Z) = dims_;
+    // The device state takes int32 edges; a wider one would be truncated.
+    if (X > (size_t)INT32_MAX || Y > (size_t)INT32_MAX || Z > (size_t)INT32_MAX)
+        throw std::invalid_argument("domain too large");
     // A constructor that throws runs no destructor: free what was built.
     try {
         rho = new Field::ScalarField<plbpd::real, 1>(Z, Y, X, 0);

@@ -171,6 +171,13 @@ void test_eqlbdist() {  // TestEqlbDist.cpp:11-35
     EXPECT(err == 0.0f);
 }
 
+// The device state takes int32 edges: a wider SimData edge is refused
+// before anything is allocated, host or device.
+void test_simdata_size_limit() {
+    EXPECT(throws<std::invalid_argument>([] { SimData s(std::make_tuple((size_t)1 << 32, 4, 4), 19); }));
+    EXPECT(throws<std::invalid_argument>([] { SimData s(std::make_tuple(4, 4, ((size_t)1 << 31) + 5), 19); }));
+}
+
 void test_flow_names() {
     EXPECT(throws<std::invalid_argument>([] { Flow f("Taylor"); }));
     Flow f("Poiseuille2D");
@@ -483,6 +490,7 @@ int main(int argc, char** argv) {
         test_domain();
         test_eqlbdist();
         test_flow_names();
+        test_simdata_size_limit();
         test_scenario_files();
     }
     if (what == "gpu" || what == "all") {

@@ -62,6 +62,15 @@ def test_oversized_domains_are_refused_before_any_device_call(mode, dims):
         P.State(19, *dims, variant=capi.F64, mode=mode)
 
 
+def test_wrapper_refuses_edges_ctypes_would_truncate():
+    """2^32 + 4 would reach the int32 descriptor as 4."""
+    with pytest.raises(P.InvalidArgument, match="too large"):
+        P.State(19, (1 << 32) + 4, 4, 4, variant=capi.F64, mode=capi.FUSED)
+    dom = {"x": 4, "y": (1 << 32) + 4, "z": 4, "tau": 0.8, "force": (0, 0, 0)}
+    with pytest.raises(P.InvalidArgument, match="too large"):
+        capi.Batch(19, capi.F32S, [dom], capi.channel())
+
+
 def test_literal_row_length_limit():
     """Literal states keep the halo'd row length X + 2 in an int."""
     with pytest.raises(P.InvalidArgument, match="too large"):
