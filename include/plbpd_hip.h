@@ -167,7 +167,12 @@ int plb_set_boundary(plb_state* s, const plb_boundary* bc, const uint8_t* mask);
 /* ---- one entry per reference call of PLBPD::march_in_time (PLBPD.cpp:24-29) */
 /* CollisionSRT::operator() (CollisionSRT.cpp:25-33, 153-176).  tau and the
  * external force F are passed directly (tau = 3 nu + 1/2 in the reference,
- * CollisionSRT.cpp:19).  Fused mode: defers to the next plb_stream. */
+ * CollisionSRT.cpp:19).  Fused mode: defers to the next plb_stream.  Literal
+ * mode (PLB_TUNE_LITERAL_FUSE on, the default): the collision runs with the
+ * next call on the state -- as one kernel with the push when that call is
+ * plb_stream(s, "push"), else before that call does anything -- so every
+ * value a caller can observe is the one the immediate collision gives; a
+ * launch failure of the collision is then reported by that next call. */
 int plb_collide_srt(plb_state* s, double tau, const double force[3]);
 /* Streaming::operator()(simdata, stream_type) (Streaming.cpp:31-42, 63-94):
  * "push" streams n into ntmp and swaps them; "pull" and anything else ->
@@ -203,7 +208,9 @@ int plb_synchronize(plb_state* s);
 
 /* ---- timers: the static per-class timers of the reference (Collision.cpp:3,
  * Streaming.cpp:12, Boundary.cpp:12-16, CalcMoments.cpp:5), measured with HIP
- * events on the state's stream.  ms_out: PLB_NSTAGES accumulated values. */
+ * events on the state's stream.  ms_out: PLB_NSTAGES accumulated values.  A
+ * literal collision run together with its push is counted under
+ * PLB_STAGE_COLLIDE; PLB_STAGE_STREAM counts the pushes run alone. */
 int plb_enable_timers(plb_state* s, int on);
 int plb_stage_times(plb_state* s, double* ms_out);
 
@@ -250,6 +257,10 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
 #define PLB_TUNE_INDEX64 7    /* 1: the cell kernels' 64-bit index instantiations even */
                               /* where 32-bit byte offsets fit (they run by themselves */
                               /* for buffers over 4 GiB); default 0                    */
+#define PLB_TUNE_LITERAL_FUSE 8 /* literal mode: 1 (default) runs a collision and the */
+                                /* push that follows it as one kernel (same values in */
+                                /* n and ntmp, one pass fewer over n); 0 runs them as */
+                                /* two kernels                                         */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
  * step will use. */

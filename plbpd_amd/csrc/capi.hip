@@ -81,7 +81,9 @@ int timer_end(plb_state* s, TimerPair* tp) {
 
 namespace {
 
-int enter(plb_state* s) {
+// Entry into a state's call without running a deferred literal collision:
+// only plb_stream uses it, to fuse that collision with its push.
+int enter_keep(plb_state* s) {
     if (!s) return fail(PLB_ERR_INVALID_ARGUMENT, "null state");
     PLB_HIP_TRY(hipSetDevice(s->device));
     return PLB_OK;
@@ -101,6 +103,21 @@ int timed(plb_state* s, int stage, Fn&& fn) {
     rc = fn();
     const int rc2 = timer_end(s, &tp);
     return rc ? rc : rc2;
+}
+
+// A literal collision that plb_collide_srt deferred runs before any other
+// call observes or changes the state; plb_stream("push") instead runs it
+// together with the push.
+int lit_flush(plb_state* s) {
+    if (!s->lit_collide_pending) return PLB_OK;
+    s->lit_collide_pending = false;
+    return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->lit_tau, s->lit_force); });
+}
+
+int enter(plb_state* s) {
+    const int rc = enter_keep(s);
+    if (rc) return rc;
+    return lit_flush(s);
 }
 
 bool same_bc(const plb_boundary& a, const plb_boundary& b) {
@@ -210,6 +227,9 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->force[0] = s->force[1] = s->force[2] = 0.0;
     s->configured = false;
     s->collide_pending = false;
+    s->lit_collide_pending = false;
+    s->lit_tau = 1.0;
+    s->lit_force[0] = s->lit_force[1] = s->lit_force[2] = 0.0;
     s->moments_valid = false;
     s->scratch = nullptr;
     s->scratch_bytes = 0;
@@ -231,6 +251,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->tune_vector = -1;
     s->tune_xcd = -1;
     s->tune_index64 = 0;
+    s->tune_lit_fuse = 1;
     s->post_collision = false;
     for (double& v : s->stage_ms) v = 0.0;
 
@@ -386,8 +407,14 @@ int plb_collide_srt(plb_state* s, double tau, const double force[3]) {
     int rc = enter(s);
     if (rc) return rc;
     if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
-    if (s->mode == PLB_MODE_LITERAL)
-        return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, tau, force); });
+    if (s->mode == PLB_MODE_LITERAL) {
+        // enter() has run any collision deferred before this one
+        if (!s->tune_lit_fuse) return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, tau, force); });
+        s->lit_collide_pending = true;
+        s->lit_tau = tau;
+        for (int i = 0; i < 3; ++i) s->lit_force[i] = force[i];
+        return PLB_OK;
+    }
     if (s->tau != tau || std::memcmp(s->force, force, sizeof(s->force)) != 0) drop_graph(s);
     s->tau = tau;
     for (int i = 0; i < 3; ++i) s->force[i] = force[i];
@@ -396,9 +423,16 @@ int plb_collide_srt(plb_state* s, double tau, const double force[3]) {
 }
 
 int plb_stream(plb_state* s, const char* stream_type) {
-    int rc = enter(s);
+    int rc = enter_keep(s);
     if (rc) return rc;
     const std::string t = stream_type ? stream_type : "";
+    if (s->lit_collide_pending && t == "push") {
+        s->lit_collide_pending = false;
+        return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, s->lit_tau, s->lit_force); });
+    }
+    // every other request finds the collision done, as the reference's does
+    rc = lit_flush(s);
+    if (rc) return rc;
     if (t != "push" && t != "pull") return fail(PLB_ERR_LOGIC, "Invalid stream type: " + t);
     if (s->mode == PLB_MODE_LITERAL) {
         if (t == "pull") return fail(PLB_ERR_LOGIC, "Streaming::_pull_tbb has not yet been implemented");
@@ -497,8 +531,12 @@ int plb_step(plb_state* s, int nsteps) {
         if (s->mode == PLB_MODE_LITERAL) {
             // PLBPD::march_in_time loop body (PLBPD.cpp:24-29)
             const plb_boundary* bc = &s->bc;
-            if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->tau, F); }))) return rc;
-            if ((rc = timed(s, PLB_STAGE_STREAM, [&] { return lit_push(s); }))) return rc;
+            if (s->tune_lit_fuse) {
+                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, s->tau, F); }))) return rc;
+            } else {
+                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->tau, F); }))) return rc;
+                if ((rc = timed(s, PLB_STAGE_STREAM, [&] { return lit_push(s); }))) return rc;
+            }
             if ((rc = timed(s, PLB_STAGE_NOSLIP, [&] { return lit_noslip(s, bc); }))) return rc;
             if ((rc = timed(s, PLB_STAGE_PERIODIC, [&] { return lit_periodic(s, bc); }))) return rc;
             if ((rc = timed(s, PLB_STAGE_MOMENTS, [&] { return lit_moments(s); }))) return rc;

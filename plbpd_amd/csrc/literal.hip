@@ -79,8 +79,37 @@ __global__ void k_init(LitDims g, R* n, R* ntmp, R* rho, R* u, double rho0, doub
     }
 }
 
-// CollisionSRT::_collision_kernel (CollisionSRT.cpp:153-176) with
-// _get_updated_u (72-81) and _get_cu (84-91), in place, every interior cell.
+// CollisionSRT::_collision_kernel (CollisionSRT.cpp:153-176) for population
+// k of one cell, with _get_cu (84-91): the post-collision value of old.
+template <int Q, class R>
+__device__ inline R collide_value(int k, R old, R rl, const R (&uu)[3], R usq, R om, R omm) {
+    const R cu = (k == 0) ? (R)0
+                          : (R)Lat<Q>::cx(k) * uu[0] + (R)Lat<Q>::cy(k) * uu[1] + (R)Lat<Q>::cz(k) * uu[2];
+    if (std::is_same<R, float>::value) {
+        const float wr = (float)Lat<Q>::w(k) * (float)rl;
+        const double neq =
+            (double)wr * (1.0 + 3.0 * (double)cu + 4.5 * (double)cu * (double)cu - 1.5 * (double)usq);
+        const float a = (float)omm * (float)old;
+        return (R)((double)a + (double)om * neq);
+    } else {
+        const double neq =
+            (Lat<Q>::w(k) * (double)rl) * (1.0 + 3.0 * (double)cu + 4.5 * (double)cu * (double)cu - 1.5 * (double)usq);
+        return (R)((double)omm * (double)old + (double)om * neq);
+    }
+}
+
+// _get_updated_u (CollisionSRT.cpp:72-81): u* = u + tau F / rho, and |u*|^2.
+template <class R>
+__device__ inline void updated_u(const LitDims& g, uint64_t c, const R* __restrict__ u, R rl, R tau, R F0, R F1,
+                                 R F2, R (&uu)[3], R& usq) {
+    const R tri = tau / rl;
+    uu[0] = u[c] + F0 * tri;
+    uu[1] = u[g.ncell + c] + F1 * tri;
+    uu[2] = u[2 * g.ncell + c] + F2 * tri;
+    usq = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
+}
+
+// CollisionSRT::operator() (CollisionSRT.cpp:25-176): in place, every interior cell.
 template <int Q, class R>
 __global__ void k_collide(LitDims g, R* __restrict__ n, const R* __restrict__ rho,
                           const R* __restrict__ u, R tau, R om, R omm, R F0, R F1, R F2) {
@@ -89,27 +118,39 @@ __global__ void k_collide(LitDims g, R* __restrict__ n, const R* __restrict__ rh
     if (x > g.X) return;
     const uint64_t c = g.cell(z, y, x);
     const R rl = rho[c];
-    const R tri = tau / rl;
-    const R uu[3] = {u[c] + F0 * tri, u[g.ncell + c] + F1 * tri, u[2 * g.ncell + c] + F2 * tri};
-    const R usq = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
+    R uu[3], usq;
+    updated_u(g, c, u, rl, tau, F0, F1, F2, uu, usq);
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        const R cu = (k == 0) ? (R)0
-                              : (R)Lat<Q>::cx(k) * uu[0] + (R)Lat<Q>::cy(k) * uu[1] +
-                                    (R)Lat<Q>::cz(k) * uu[2];
         R* p = n + k * g.ncell + c;
-        if (std::is_same<R, float>::value) {
-            const float wr = (float)Lat<Q>::w(k) * (float)rl;
-            const double neq = (double)wr * (1.0 + 3.0 * (double)cu + 4.5 * (double)cu * (double)cu -
-                                             1.5 * (double)usq);
-            const float a = (float)omm * (float)*p;
-            *p = (R)((double)a + (double)om * neq);
-        } else {
-            const double neq = (Lat<Q>::w(k) * (double)rl) *
-                               (1.0 + 3.0 * (double)cu + 4.5 * (double)cu * (double)cu -
-                                1.5 * (double)usq);
-            *p = (R)((double)omm * (double)*p + (double)om * neq);
-        }
+        *p = collide_value<Q, R>(k, *p, rl, uu, usq, om, omm);
+    }
+}
+
+// The collision followed by Streaming::_push_kernel (Streaming.cpp:63-74) in
+// one pass: each post-collision value is stored in place, as k_collide leaves
+// it (it becomes ntmp after the swap, and its entries that the next push
+// does not overwrite are read again), and at x + c_k in t, as k_push copies
+// it.  Both buffers end exactly as after k_collide then k_push; the push's
+// re-read of n is saved.
+template <int Q, class R>
+__global__ void k_collide_push(LitDims g, R* __restrict__ n, const R* __restrict__ rho,
+                               const R* __restrict__ u, R* __restrict__ t, R tau, R om, R omm, R F0, R F1,
+                               R F2) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int y = blockIdx.y + 1, z = blockIdx.z + 1;
+    if (x > g.X) return;
+    const uint64_t c = g.cell(z, y, x);
+    const R rl = rho[c];
+    R uu[3], usq;
+    updated_u(g, c, u, rl, tau, F0, F1, F2, uu, usq);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const R v = collide_value<Q, R>(k, n[k * g.ncell + c], rl, uu, usq, om, omm);
+        n[k * g.ncell + c] = v;
+        const int64_t off = (int64_t)Lat<Q>::cx(k) + (int64_t)Lat<Q>::cy(k) * g.nx +
+                            (int64_t)Lat<Q>::cz(k) * g.nx * g.ny;
+        t[k * g.ncell + (uint64_t)((int64_t)c + off)] = v;
     }
 }
 
@@ -384,26 +425,49 @@ int lit_initialize(plb_state* s, double rho0, const double u0[3], const plb_init
     });
 }
 
+namespace {
+// tau and omega = 1/tau as CollisionSRT.cpp:19-20 hold them, and 1 - omega.
+template <class R>
+void relaxation(double tau, R* tr, R* om, R* omm) {
+    *tr = (R)tau;
+    if (std::is_same<R, float>::value)
+        *om = (R)(1. / (double)*tr);
+    else
+        *om = (R)(1.0 / tau);
+    *omm = (R)1 - *om;
+}
+}  // namespace
+
 int lit_collide(plb_state* s, double tau, const double F[3]) {
     const LitDims g = dims_of(s);
     return with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
         using R = decltype(rr);
         R tr, om, omm;
-        if (std::is_same<R, float>::value) {  // CollisionSRT.cpp:19-20
-            tr = (R)tau;
-            om = (R)(1. / (double)tr);
-        } else {
-            tr = (R)tau;
-            om = (R)(1.0 / tau);
-        }
-        omm = (R)1 - om;
+        relaxation(tau, &tr, &om, &omm);
         hipLaunchKernelGGL((k_collide<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
                            (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, tr, om, omm,
                            (R)F[0], (R)F[1], (R)F[2]);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
+}
+
+int lit_collide_push(plb_state* s, double tau, const double F[3]) {
+    const LitDims g = dims_of(s);
+    const int rc = with_qr(s, [&](auto qc, auto rr) {
+        constexpr int Q = decltype(qc)::value;
+        using R = decltype(rr);
+        R tr, om, omm;
+        relaxation(tau, &tr, &om, &omm);
+        hipLaunchKernelGGL((k_collide_push<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
+                           (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, (R*)s->buf[s->cur ^ 1], tr, om,
+                           omm, (R)F[0], (R)F[1], (R)F[2]);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+    if (rc == PLB_OK) s->cur ^= 1;  // std::swap(simdata.ntmp, simdata.n)
+    return rc;
 }
 
 int lit_push(plb_state* s) {

@@ -68,7 +68,10 @@ struct plb_state {
     double tau;
     double force[3];
     bool configured;
-    bool collide_pending;
+    bool collide_pending;      // fused: the next fused launch collides first
+    bool lit_collide_pending;  // literal: a collision waits for the push (or any other call)
+    double lit_tau;            // ... with these arguments
+    double lit_force[3];
     bool moments_valid;
     void* scratch;  // reductions, face staging
     size_t scratch_bytes;
@@ -85,6 +88,7 @@ struct plb_state {
     int tune_xcd;         // tile mapping: 0 block i -> tile i, 1 XCD runs, 2 XCD y-stripes, -1 auto
     int tune_index64;     // 1: 64-bit index kernels even where 32-bit offsets fit
     int tune_vector;      // row pull kernel cells per lane (1, 2, 4) where X allows; 0 off; -1 auto
+    int tune_lit_fuse;    // literal: 1 collision and push in one kernel when they follow each other
     bool post_collision;  // fused: buf[cur] holds post-collision populations (pull step)
     // timers
     bool timers_on;
@@ -104,6 +108,7 @@ int lit_initialize(plb_state* s, double rho0, const double u0[3], const plb_init
                    const double* tg_dev);
 int lit_collide(plb_state* s, double tau, const double F[3]);
 int lit_push(plb_state* s);
+int lit_collide_push(plb_state* s, double tau, const double F[3]);
 int lit_noslip(plb_state* s, const plb_boundary* bc);
 int lit_periodic(plb_state* s, const plb_boundary* bc);
 int lit_moments(plb_state* s);

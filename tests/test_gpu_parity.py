@@ -81,9 +81,73 @@ def test_literal_stages_bitexact(orc, q, X, Y, Z, prec):
 
 
 @pytest.mark.parametrize("q,X,Y,Z", LITERAL_CASES)
-def test_literal_run_bitexact(orc, q, X, Y, Z):
-    """plb_step (the whole loop body) over 40 steps, FP64."""
-    print(f"\n[case] literal run q={q} {X}x{Y}x{Z}", flush=True)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_literal_collide_then_push_bitexact(orc, q, X, Y, Z, prec):
+    """The staged calls with nothing between collide and push, so the library
+    runs them as one kernel: n, ntmp, rho and u after every step equal the
+    oracle's separate collide and push (ntmp holds the post-collision values
+    the push read)."""
+    print(f"\n[case] literal collide+push q={q} {X}x{Y}x{Z} {prec}", flush=True)
+    dt = np.float64 if prec == "f64" else np.float32
+    types = couette_types(q)
+    ob, pb = both_bc(types, {UP: (0.05, 0.01 if q != 9 else 0.0, 0.0), DOWN: (0.0, 0.0, 0.0)})
+    tau, F = 0.7, (1e-4, -2e-5 if q != 9 else 0.0, 3e-5)
+    o = orc.lit_state(q, X, Y, Z, dt)
+    orc.lit_init(o, 1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 99, 1e-2), ob)
+    with P.State(q, X, Y, Z, prec, P.LITERAL) as st:
+        st.initialize(1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 99, 1e-2), pb)
+        for step in range(3):
+            orc.lit_collide(o, tau, F)
+            orc.lit_push(o)
+            st.collide(tau, F)
+            st.stream("push")
+            lit_compare(st, o, f"step {step} collide+push")
+            assert orc.lit_noslip(o, ob) == 0
+            orc.lit_periodic(o, ob)
+            orc.lit_moments(o)
+            orc.lit_reset(o, ob)
+            st.apply_noslip(pb)
+            st.apply_periodicity(pb)
+            st.calc_moments()
+            st.reset(pb)
+            lit_compare(st, o, f"step {step} end")
+
+
+def test_literal_deferred_collision_runs_before_other_calls(orc):
+    """A literal collision waits only for a directly following push.  Two
+    collisions in a row both run; a stream request the literal mode refuses
+    ("pull") still finds the collision done, as in the reference, where
+    the collision ran before Streaming threw (Streaming.cpp:96-102)."""
+    q, X, Y, Z = 19, 6, 5, 4
+    ob, pb = both_bc(couette_types(q), {UP: (0.03, 0.0, 0.0), DOWN: (0.0, 0.0, 0.0)})
+    tau, F = 0.8, (2e-5, 0.0, 0.0)
+    o = orc.lit_state(q, X, Y, Z)
+    orc.lit_init(o, 1.0, (0, 0, 0), (P.INIT_PERTURBED, 5, 1e-2), ob)
+    with P.State(q, X, Y, Z, "f64", P.LITERAL) as st:
+        st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 5, 1e-2), pb)
+        orc.lit_collide(o, tau, F)
+        orc.lit_collide(o, tau, F)
+        orc.lit_push(o)
+        st.collide(tau, F)
+        st.collide(tau, F)
+        st.stream("push")
+        lit_compare(st, o, "collide, collide, push")
+        orc.lit_collide(o, tau, F)
+        st.collide(tau, F)
+        with pytest.raises(P.LogicError):
+            st.stream("pull")
+        lit_compare(st, o, "collide, refused pull")
+        orc.lit_push(o)
+        st.stream("push")
+        lit_compare(st, o, "then push")
+
+
+@pytest.mark.parametrize("q,X,Y,Z", LITERAL_CASES)
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_literal_run_bitexact(orc, q, X, Y, Z, fuse):
+    """plb_step (the whole loop body) over 40 steps, FP64, with collision and
+    push in one kernel (the default) and in two."""
+    print(f"\n[case] literal run q={q} {X}x{Y}x{Z} fuse={fuse}", flush=True)
     types = couette_types(q)
     ob, pb = both_bc(types, {UP: (0.02, 0.0, 0.0), DOWN: (0.0, 0.0, 0.0)})
     tau, F = 0.9, (1e-5, 0.0, 0.0)
@@ -91,6 +155,7 @@ def test_literal_run_bitexact(orc, q, X, Y, Z):
     orc.lit_init(o, 1.0, (0, 0, 0), (P.INIT_PERTURBED, 7, 1e-3), ob)
     orc.lit_run(o, tau, F, ob, 40)
     with P.State(q, X, Y, Z, "f64", P.LITERAL) as st:
+        st.set_tuning(P.capi.TUNE_LITERAL_FUSE, fuse)
         st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 7, 1e-3), pb)
         st.configure(tau, F, pb)
         st.step(40)
@@ -468,13 +533,18 @@ def test_error_codes():
                                        {WEST: (0, 0, 0)}))
 
 
-def test_timers_accumulate():
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_timers_accumulate(fuse):
+    """Per-stage HIP-event times of the literal loop.  A collision run with
+    its push in one kernel counts under collide; stream then stays at zero."""
     _, pb = fused_bc(19, True)
     with P.State(19, 8, 8, 8, "f64", P.LITERAL) as st:
+        st.set_tuning(P.capi.TUNE_LITERAL_FUSE, fuse)
         st.initialize(1.0, (0, 0, 0), None, pb)
         st.configure(0.8, (1e-5, 0, 0), pb)
         st.enable_timers(True)
         st.step(5)
         t = st.stage_times()
-    for k in ("collide", "stream", "noslip", "periodicity", "moments"):
+    for k in ("collide", "noslip", "periodicity", "moments"):
         assert t[k] > 0.0
+    assert (t["stream"] > 0.0) == (fuse == 0)
