@@ -210,7 +210,9 @@ int plb_synchronize(plb_state* s);
  * Streaming.cpp:12, Boundary.cpp:12-16, CalcMoments.cpp:5), measured with HIP
  * events on the state's stream.  ms_out: PLB_NSTAGES accumulated values.  A
  * literal collision run together with its push is counted under
- * PLB_STAGE_COLLIDE; PLB_STAGE_STREAM counts the pushes run alone. */
+ * PLB_STAGE_COLLIDE; PLB_STAGE_STREAM counts the pushes run alone.  Inside a
+ * literal plb_step the moments and resets of every step but the last are
+ * counted under PLB_STAGE_COLLIDE too. */
 int plb_enable_timers(plb_state* s, int on);
 int plb_stage_times(plb_state* s, double* ms_out);
 
@@ -259,8 +261,10 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                               /* for buffers over 4 GiB); default 0                    */
 #define PLB_TUNE_LITERAL_FUSE 8 /* literal mode: 1 (default) runs a collision and the */
                                 /* push that follows it as one kernel (same values in */
-                                /* n and ntmp, one pass fewer over n); 0 runs them as */
-                                /* two kernels                                         */
+                                /* n and ntmp, one pass fewer over n), and inside     */
+                                /* plb_step computes each step's moments and resets  */
+                                /* in the next step's kernel (same rho and u); 0 runs */
+                                /* one kernel per reference call                       */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
  * step will use. */

@@ -529,9 +529,16 @@ int plb_step(plb_state* s, int nsteps) {
     const double* F = s->force;
     for (int i = 0; i < nsteps; ++i) {
         if (s->mode == PLB_MODE_LITERAL) {
-            // PLBPD::march_in_time loop body (PLBPD.cpp:24-29)
+            // PLBPD::march_in_time loop body (PLBPD.cpp:24-29).  With the
+            // kernels folded, the moments and resets of every step but the
+            // last are computed by the next step's collision kernel, which
+            // stores rho and u as those stages would before it collides.
             const plb_boundary* bc = &s->bc;
-            if (s->tune_lit_fuse) {
+            const bool fold = s->tune_lit_fuse != 0;
+            if (fold && i > 0) {
+                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_moments_collide_push(s, s->tau, F, bc); })))
+                    return rc;
+            } else if (fold) {
                 if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, s->tau, F); }))) return rc;
             } else {
                 if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->tau, F); }))) return rc;
@@ -539,6 +546,7 @@ int plb_step(plb_state* s, int nsteps) {
             }
             if ((rc = timed(s, PLB_STAGE_NOSLIP, [&] { return lit_noslip(s, bc); }))) return rc;
             if ((rc = timed(s, PLB_STAGE_PERIODIC, [&] { return lit_periodic(s, bc); }))) return rc;
+            if (fold && i + 1 < nsteps) continue;
             if ((rc = timed(s, PLB_STAGE_MOMENTS, [&] { return lit_moments(s); }))) return rc;
             if ((rc = timed(s, PLB_STAGE_RESET_VELOCITY, [&] { return lit_reset_velocity(s, bc); }))) return rc;
             if ((rc = timed(s, PLB_STAGE_RESET_DENSITY, [&] { return lit_reset_density(s, bc); }))) return rc;

@@ -100,12 +100,11 @@ __device__ inline R collide_value(int k, R old, R rl, const R (&uu)[3], R usq, R
 
 // _get_updated_u (CollisionSRT.cpp:72-81): u* = u + tau F / rho, and |u*|^2.
 template <class R>
-__device__ inline void updated_u(const LitDims& g, uint64_t c, const R* __restrict__ u, R rl, R tau, R F0, R F1,
-                                 R F2, R (&uu)[3], R& usq) {
+__device__ inline void updated_u(R u0, R u1, R u2, R rl, R tau, R F0, R F1, R F2, R (&uu)[3], R& usq) {
     const R tri = tau / rl;
-    uu[0] = u[c] + F0 * tri;
-    uu[1] = u[g.ncell + c] + F1 * tri;
-    uu[2] = u[2 * g.ncell + c] + F2 * tri;
+    uu[0] = u0 + F0 * tri;
+    uu[1] = u1 + F1 * tri;
+    uu[2] = u2 + F2 * tri;
     usq = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
 }
 
@@ -119,7 +118,7 @@ __global__ void k_collide(LitDims g, R* __restrict__ n, const R* __restrict__ rh
     const uint64_t c = g.cell(z, y, x);
     const R rl = rho[c];
     R uu[3], usq;
-    updated_u(g, c, u, rl, tau, F0, F1, F2, uu, usq);
+    updated_u(u[c], u[g.ncell + c], u[2 * g.ncell + c], rl, tau, F0, F1, F2, uu, usq);
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         R* p = n + k * g.ncell + c;
@@ -143,7 +142,7 @@ __global__ void k_collide_push(LitDims g, R* __restrict__ n, const R* __restrict
     const uint64_t c = g.cell(z, y, x);
     const R rl = rho[c];
     R uu[3], usq;
-    updated_u(g, c, u, rl, tau, F0, F1, F2, uu, usq);
+    updated_u(u[c], u[g.ncell + c], u[2 * g.ncell + c], rl, tau, F0, F1, F2, uu, usq);
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const R v = collide_value<Q, R>(k, n[k * g.ncell + c], rl, uu, usq, om, omm);
@@ -286,6 +285,26 @@ __global__ void k_periodic(LitDims g, int axis, R* __restrict__ n) {
     n[k * g.ncell + dst] = n[k * g.ncell + src];
 }
 
+// CalcMoments::_get_local_moments (CalcMoments.hpp:22-37): rho = sum_k n_k,
+// u = (sum_k n_k c_k) * (1 / rho), k ascending.
+template <int Q, class R>
+struct MomentSum {
+    R rl = (R)0, u0 = (R)0, u1 = (R)0, u2 = (R)0;
+    __device__ void add(int k, R nk) {
+        rl += nk;
+        u0 += nk * (R)Lat<Q>::cx(k);
+        u1 += nk * (R)Lat<Q>::cy(k);
+        u2 += nk * (R)Lat<Q>::cz(k);
+    }
+    __device__ void finish(R& rho, R (&u)[3]) const {
+        const R ri = (R)1 / rl;
+        rho = rl;
+        u[0] = u0 * ri;
+        u[1] = u1 * ri;
+        u[2] = u2 * ri;
+    }
+};
+
 // CalcMoments::_get_local_moments (CalcMoments.hpp:22-37) on interior cells.
 // The populations are read with non-temporal loads (measured 3-9% faster
 // for this stage on MI355X).
@@ -296,20 +315,15 @@ __global__ void k_moments(LitDims g, const R* __restrict__ n, R* __restrict__ rh
     const int y = blockIdx.y + 1, z = blockIdx.z + 1;
     if (x > g.X) return;
     const uint64_t c = g.cell(z, y, x);
-    R rl = (R)0, u0 = (R)0, u1 = (R)0, u2 = (R)0;
+    MomentSum<Q, R> m;
 #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const R nk = __builtin_nontemporal_load(n + k * g.ncell + c);
-        rl += nk;
-        u0 += nk * (R)Lat<Q>::cx(k);
-        u1 += nk * (R)Lat<Q>::cy(k);
-        u2 += nk * (R)Lat<Q>::cz(k);
-    }
-    const R ri = (R)1 / rl;
+    for (int k = 0; k < Q; ++k) m.add(k, __builtin_nontemporal_load(n + k * g.ncell + c));
+    R rl, um[3];
+    m.finish(rl, um);
     rho[c] = rl;
-    u[c] = u0 * ri;
-    u[g.ncell + c] = u1 * ri;
-    u[2 * g.ncell + c] = u2 * ri;
+    u[c] = um[0];
+    u[g.ncell + c] = um[1];
+    u[2 * g.ncell + c] = um[2];
 }
 
 // Boundary::_set_velocity / _set_density (Boundary.cpp:150-187) on one face.
@@ -325,6 +339,73 @@ __global__ void k_set_face(LitDims g, FaceBox b, R* __restrict__ dst, int vlen, 
     if (vlen == 3) {
         dst[g.ncell + c] = v1;
         dst[2 * g.ncell + c] = v2;
+    }
+}
+
+// Boundary::reset (Boundary.cpp:58-89, 150-187) for one interior cell: the
+// faces in order, a later face overwriting an earlier one, as the per-face
+// k_set_face launches of lit_reset_velocity and lit_reset_density do.  Face f
+// is the layer at coordinate X|Y|Z (f even) or 1 (f odd) on axis f / 2, all
+// interior cells of the other two axes (Boundary.cpp:113-148).
+template <class R>
+struct FaceResets {
+    int has_vel[6], noslip[6];
+    R vel[6][3];
+    R rho_s;
+};
+
+template <class R>
+__device__ inline void apply_resets(const FaceResets<R>& fr, const LitDims& g, int x, int y, int z, R& rho,
+                                    R (&u)[3]) {
+    const int co[3] = {x, y, z}, dim[3] = {g.X, g.Y, g.Z};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        if (!fr.has_vel[f] || co[f >> 1] != ((f & 1) == 0 ? dim[f >> 1] : 1)) continue;
+        u[0] = fr.vel[f][0];
+        u[1] = fr.vel[f][1];
+        u[2] = fr.vel[f][2];
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+        if (fr.noslip[f] && co[f >> 1] == ((f & 1) == 0 ? dim[f >> 1] : 1)) rho = fr.rho_s;
+}
+
+// Inside the literal step loop: the previous step's moments and resets
+// (CalcMoments, Boundary::reset) folded into this step's collision and push.
+// rho and u are computed from n exactly as k_moments and the resets leave
+// them, stored (no-slip reads rho before the next moments), and used by the
+// collision at once; n and t end as after k_collide_push.
+template <int Q, class R>
+__global__ void k_moments_collide_push(LitDims g, R* __restrict__ n, R* __restrict__ rho, R* __restrict__ u,
+                                       R* __restrict__ t, FaceResets<R> fr, R tau, R om, R omm, R F0, R F1,
+                                       R F2) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int y = blockIdx.y + 1, z = blockIdx.z + 1;
+    if (x > g.X) return;
+    const uint64_t c = g.cell(z, y, x);
+    R nv[Q];
+    MomentSum<Q, R> m;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        nv[k] = n[k * g.ncell + c];
+        m.add(k, nv[k]);
+    }
+    R rl, um[3];
+    m.finish(rl, um);
+    apply_resets(fr, g, x, y, z, rl, um);
+    rho[c] = rl;
+    u[c] = um[0];
+    u[g.ncell + c] = um[1];
+    u[2 * g.ncell + c] = um[2];
+    R uu[3], usq;
+    updated_u(um[0], um[1], um[2], rl, tau, F0, F1, F2, uu, usq);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const R v = collide_value<Q, R>(k, nv[k], rl, uu, usq, om, omm);
+        n[k * g.ncell + c] = v;
+        const int64_t off = (int64_t)Lat<Q>::cx(k) + (int64_t)Lat<Q>::cy(k) * g.nx +
+                            (int64_t)Lat<Q>::cz(k) * g.nx * g.ny;
+        t[k * g.ncell + (uint64_t)((int64_t)c + off)] = v;
     }
 }
 
@@ -463,6 +544,30 @@ int lit_collide_push(plb_state* s, double tau, const double F[3]) {
         hipLaunchKernelGGL((k_collide_push<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
                            (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, (R*)s->buf[s->cur ^ 1], tr, om,
                            omm, (R)F[0], (R)F[1], (R)F[2]);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+    if (rc == PLB_OK) s->cur ^= 1;  // std::swap(simdata.ntmp, simdata.n)
+    return rc;
+}
+
+int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc) {
+    const LitDims g = dims_of(s);
+    const int rc = with_qr(s, [&](auto qc, auto rr) {
+        constexpr int Q = decltype(qc)::value;
+        using R = decltype(rr);
+        R tr, om, omm;
+        relaxation(tau, &tr, &om, &omm);
+        FaceResets<R> fr;
+        for (int f = 0; f < 6; ++f) {
+            fr.has_vel[f] = bc->has_vel[f] != 0;
+            fr.noslip[f] = bc->type[f] == kNoslip;
+            for (int i = 0; i < 3; ++i) fr.vel[f][i] = (R)bc->vel[f][i];
+        }
+        fr.rho_s = (R)bc->rho_s;
+        hipLaunchKernelGGL((k_moments_collide_push<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream,
+                           g, (R*)s->buf[s->cur], (R*)s->rho, (R*)s->u, (R*)s->buf[s->cur ^ 1], fr, tr, om, omm,
+                           (R)F[0], (R)F[1], (R)F[2]);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });

@@ -109,6 +109,7 @@ int lit_initialize(plb_state* s, double rho0, const double u0[3], const plb_init
 int lit_collide(plb_state* s, double tau, const double F[3]);
 int lit_push(plb_state* s);
 int lit_collide_push(plb_state* s, double tau, const double F[3]);
+int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc);
 int lit_noslip(plb_state* s, const plb_boundary* bc);
 int lit_periodic(plb_state* s, const plb_boundary* bc);
 int lit_moments(plb_state* s);
