@@ -185,9 +185,15 @@ int plb_stream(plb_state* s, const char* stream_type);
 int plb_apply_noslip(plb_state* s, const plb_boundary* bc);
 /* Boundary::apply_periodicity (Boundary.cpp:91-99, 223-281). */
 int plb_apply_periodicity(plb_state* s, const plb_boundary* bc);
-/* CalcMoments::operator() (CalcMoments.hpp:46-73). */
+/* CalcMoments::operator() (CalcMoments.hpp:46-73).  Literal mode with
+ * PLB_TUNE_LITERAL_FUSE on: the moments, and the resets that directly follow
+ * them, run with the next call on the state -- folded into the collide+push
+ * kernel when that sequence is collide then plb_stream(s, "push"), else
+ * before the call does anything -- so every observable value is unchanged. */
 int plb_calc_moments(plb_state* s);
-/* Boundary::reset / reset_velocity / reset_density (Boundary.cpp:58-89). */
+/* Boundary::reset / reset_velocity / reset_density (Boundary.cpp:58-89).
+ * Deferred in literal mode directly after plb_calc_moments, as described
+ * there. */
 int plb_reset(plb_state* s, const plb_boundary* bc);
 int plb_reset_velocity(plb_state* s, const plb_boundary* bc);
 int plb_reset_density(plb_state* s, const plb_boundary* bc);
@@ -210,9 +216,10 @@ int plb_synchronize(plb_state* s);
  * Streaming.cpp:12, Boundary.cpp:12-16, CalcMoments.cpp:5), measured with HIP
  * events on the state's stream.  ms_out: PLB_NSTAGES accumulated values.  A
  * literal collision run together with its push is counted under
- * PLB_STAGE_COLLIDE; PLB_STAGE_STREAM counts the pushes run alone.  Inside a
- * literal plb_step the moments and resets of every step but the last are
- * counted under PLB_STAGE_COLLIDE too. */
+ * PLB_STAGE_COLLIDE; PLB_STAGE_STREAM counts the pushes run alone.  Moments
+ * and resets folded into the next step's kernel (inside a literal plb_step,
+ * or deferred as plb_calc_moments describes) are counted under
+ * PLB_STAGE_COLLIDE too. */
 int plb_enable_timers(plb_state* s, int on);
 int plb_stage_times(plb_state* s, double* ms_out);
 
@@ -261,10 +268,11 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                               /* for buffers over 4 GiB); default 0                    */
 #define PLB_TUNE_LITERAL_FUSE 8 /* literal mode: 1 (default) runs a collision and the */
                                 /* push that follows it as one kernel (same values in */
-                                /* n and ntmp, one pass fewer over n), and inside     */
-                                /* plb_step computes each step's moments and resets  */
-                                /* in the next step's kernel (same rho and u); 0 runs */
-                                /* one kernel per reference call                       */
+                                /* n and ntmp, one pass fewer over n), and computes   */
+                                /* each step's moments and resets in the next step's */
+                                /* kernel (same rho and u), inside plb_step and for   */
+                                /* the staged calls alike; 0 runs one kernel per      */
+                                /* reference call                                      */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
  * step will use. */

@@ -1,6 +1,7 @@
 // C ABI of the plbpd HIP library: state lifecycle, one entry point per
 // reference operator call, fused stepping, HIP-graph replay, timers.
 #include <cmath>
+#include <initializer_list>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -105,13 +106,46 @@ int timed(plb_state* s, int stage, Fn&& fn) {
     return rc ? rc : rc2;
 }
 
-// A literal collision that plb_collide_srt deferred runs before any other
-// call observes or changes the state; plb_stream("push") instead runs it
-// together with the push.
+// Literal calls deferred in s->lit_queue run, in order, before any other call
+// observes or changes the state; plb_stream("push") instead folds them into
+// its kernel.
 int lit_flush(plb_state* s) {
-    if (!s->lit_collide_pending) return PLB_OK;
-    s->lit_collide_pending = false;
-    return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->lit_tau, s->lit_force); });
+    std::vector<LitOp> q;
+    q.swap(s->lit_queue);
+    for (const LitOp& op : q) {
+        int rc;
+        switch (op.kind) {
+            case kLitMoments: rc = timed(s, PLB_STAGE_MOMENTS, [&] { return lit_moments(s); }); break;
+            case kLitResetVelocity:
+                rc = timed(s, PLB_STAGE_RESET_VELOCITY, [&] { return lit_reset_velocity(s, &op.bc); });
+                break;
+            case kLitResetDensity:
+                rc = timed(s, PLB_STAGE_RESET_DENSITY, [&] { return lit_reset_density(s, &op.bc); });
+                break;
+            default: rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, op.tau, op.force); }); break;
+        }
+        if (rc) return rc;
+    }
+    return PLB_OK;
+}
+
+bool lit_queue_is(const plb_state* s, std::initializer_list<LitOpKind> kinds) {
+    if (s->lit_queue.size() != kinds.size()) return false;
+    size_t i = 0;
+    for (LitOpKind k : kinds)
+        if (s->lit_queue[i++].kind != k) return false;
+    return true;
+}
+
+LitOp lit_op(LitOpKind kind, const plb_boundary* bc = nullptr, double tau = 0.0, const double* force = nullptr) {
+    LitOp op;
+    std::memset(&op, 0, sizeof(op));
+    op.kind = kind;
+    if (bc) op.bc = *bc;
+    op.tau = tau;
+    if (force)
+        for (int i = 0; i < 3; ++i) op.force[i] = force[i];
+    return op;
 }
 
 int enter(plb_state* s) {
@@ -227,9 +261,6 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->force[0] = s->force[1] = s->force[2] = 0.0;
     s->configured = false;
     s->collide_pending = false;
-    s->lit_collide_pending = false;
-    s->lit_tau = 1.0;
-    s->lit_force[0] = s->lit_force[1] = s->lit_force[2] = 0.0;
     s->moments_valid = false;
     s->scratch = nullptr;
     s->scratch_bytes = 0;
@@ -404,17 +435,20 @@ int plb_set_boundary(plb_state* s, const plb_boundary* bc, const uint8_t* mask) 
 }
 
 int plb_collide_srt(plb_state* s, double tau, const double force[3]) {
-    int rc = enter(s);
+    int rc = enter_keep(s);
     if (rc) return rc;
-    if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
     if (s->mode == PLB_MODE_LITERAL) {
-        // enter() has run any collision deferred before this one
+        // a collision waits for the push after nothing or after the moments
+        // and resets of the step before
+        if (!s->tune_lit_fuse ||
+            !(s->lit_queue.empty() || lit_queue_is(s, {kLitMoments, kLitResetVelocity, kLitResetDensity})))
+            if ((rc = lit_flush(s))) return rc;
+        if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
         if (!s->tune_lit_fuse) return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, tau, force); });
-        s->lit_collide_pending = true;
-        s->lit_tau = tau;
-        for (int i = 0; i < 3; ++i) s->lit_force[i] = force[i];
+        s->lit_queue.push_back(lit_op(kLitCollide, nullptr, tau, force));
         return PLB_OK;
     }
+    if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
     if (s->tau != tau || std::memcmp(s->force, force, sizeof(s->force)) != 0) drop_graph(s);
     s->tau = tau;
     for (int i = 0; i < 3; ++i) s->force[i] = force[i];
@@ -426,11 +460,18 @@ int plb_stream(plb_state* s, const char* stream_type) {
     int rc = enter_keep(s);
     if (rc) return rc;
     const std::string t = stream_type ? stream_type : "";
-    if (s->lit_collide_pending && t == "push") {
-        s->lit_collide_pending = false;
-        return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, s->lit_tau, s->lit_force); });
+    if (t == "push" && lit_queue_is(s, {kLitMoments, kLitResetVelocity, kLitResetDensity, kLitCollide})) {
+        const std::vector<LitOp> q = std::move(s->lit_queue);
+        s->lit_queue.clear();
+        return timed(s, PLB_STAGE_COLLIDE,
+                     [&] { return lit_moments_collide_push(s, q[3].tau, q[3].force, &q[1].bc, &q[2].bc); });
     }
-    // every other request finds the collision done, as the reference's does
+    if (t == "push" && lit_queue_is(s, {kLitCollide})) {
+        const LitOp c = s->lit_queue[0];
+        s->lit_queue.clear();
+        return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, c.tau, c.force); });
+    }
+    // every other request finds the deferred calls done, as the reference's does
     rc = lit_flush(s);
     if (rc) return rc;
     if (t != "push" && t != "pull") return fail(PLB_ERR_LOGIC, "Invalid stream type: " + t);
@@ -470,14 +511,23 @@ int plb_apply_periodicity(plb_state* s, const plb_boundary* bc) {
 int plb_calc_moments(plb_state* s) {
     int rc = enter(s);
     if (rc) return rc;
-    if (s->mode == PLB_MODE_LITERAL) return timed(s, PLB_STAGE_MOMENTS, [&] { return lit_moments(s); });
+    if (s->mode == PLB_MODE_LITERAL) {
+        if (!s->tune_lit_fuse) return timed(s, PLB_STAGE_MOMENTS, [&] { return lit_moments(s); });
+        s->lit_queue.push_back(lit_op(kLitMoments));  // enter() left the queue empty
+        return PLB_OK;
+    }
     s->moments_valid = false;  // computed on demand from the populations
     return PLB_OK;
 }
 
 int plb_reset_velocity(plb_state* s, const plb_boundary* bc) {
-    int rc = enter(s);
+    int rc = enter_keep(s);
     if (rc) return rc;
+    if (s->tune_lit_fuse && bc && lit_queue_is(s, {kLitMoments})) {
+        s->lit_queue.push_back(lit_op(kLitResetVelocity, bc));
+        return PLB_OK;
+    }
+    if ((rc = lit_flush(s))) return rc;
     if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
     if (s->mode == PLB_MODE_LITERAL)
         return timed(s, PLB_STAGE_RESET_VELOCITY, [&] { return lit_reset_velocity(s, bc); });
@@ -485,8 +535,13 @@ int plb_reset_velocity(plb_state* s, const plb_boundary* bc) {
 }
 
 int plb_reset_density(plb_state* s, const plb_boundary* bc) {
-    int rc = enter(s);
+    int rc = enter_keep(s);
     if (rc) return rc;
+    if (s->tune_lit_fuse && bc && lit_queue_is(s, {kLitMoments, kLitResetVelocity})) {
+        s->lit_queue.push_back(lit_op(kLitResetDensity, bc));
+        return PLB_OK;
+    }
+    if ((rc = lit_flush(s))) return rc;
     if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
     if (s->mode == PLB_MODE_LITERAL)
         return timed(s, PLB_STAGE_RESET_DENSITY, [&] { return lit_reset_density(s, bc); });
@@ -536,7 +591,8 @@ int plb_step(plb_state* s, int nsteps) {
             const plb_boundary* bc = &s->bc;
             const bool fold = s->tune_lit_fuse != 0;
             if (fold && i > 0) {
-                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_moments_collide_push(s, s->tau, F, bc); })))
+                if ((rc = timed(s, PLB_STAGE_COLLIDE,
+                                [&] { return lit_moments_collide_push(s, s->tau, F, bc, bc); })))
                     return rc;
             } else if (fold) {
                 if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, s->tau, F); }))) return rc;

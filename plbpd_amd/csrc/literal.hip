@@ -551,7 +551,8 @@ int lit_collide_push(plb_state* s, double tau, const double F[3]) {
     return rc;
 }
 
-int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc) {
+int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc_vel,
+                             const plb_boundary* bc_rho) {
     const LitDims g = dims_of(s);
     const int rc = with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
@@ -560,11 +561,11 @@ int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const 
         relaxation(tau, &tr, &om, &omm);
         FaceResets<R> fr;
         for (int f = 0; f < 6; ++f) {
-            fr.has_vel[f] = bc->has_vel[f] != 0;
-            fr.noslip[f] = bc->type[f] == kNoslip;
-            for (int i = 0; i < 3; ++i) fr.vel[f][i] = (R)bc->vel[f][i];
+            fr.has_vel[f] = bc_vel->has_vel[f] != 0;
+            fr.noslip[f] = bc_rho->type[f] == kNoslip;
+            for (int i = 0; i < 3; ++i) fr.vel[f][i] = (R)bc_vel->vel[f][i];
         }
-        fr.rho_s = (R)bc->rho_s;
+        fr.rho_s = (R)bc_rho->rho_s;
         hipLaunchKernelGGL((k_moments_collide_push<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream,
                            g, (R*)s->buf[s->cur], (R*)s->rho, (R*)s->u, (R*)s->buf[s->cur ^ 1], fr, tr, om, omm,
                            (R)F[0], (R)F[1], (R)F[2]);

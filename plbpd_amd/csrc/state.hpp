@@ -47,6 +47,15 @@ struct TimerPair {
     hipEvent_t start, stop;
 };
 
+// ------------------------------------------------ deferred literal calls ---
+enum LitOpKind { kLitMoments, kLitResetVelocity, kLitResetDensity, kLitCollide };
+struct LitOp {
+    LitOpKind kind;
+    double tau;        // kLitCollide
+    double force[3];   // kLitCollide
+    plb_boundary bc;   // the resets
+};
+
 }  // namespace plb
 
 struct plb_state {
@@ -69,9 +78,10 @@ struct plb_state {
     double force[3];
     bool configured;
     bool collide_pending;      // fused: the next fused launch collides first
-    bool lit_collide_pending;  // literal: a collision waits for the push (or any other call)
-    double lit_tau;            // ... with these arguments
-    double lit_force[3];
+    // literal: calls deferred so that a following push can fold them into
+    // its kernel (at most moments, reset velocity, reset density, collide, in
+    // that order); any other call runs them first, in order
+    std::vector<plb::LitOp> lit_queue;
     bool moments_valid;
     void* scratch;  // reductions, face staging
     size_t scratch_bytes;
@@ -109,7 +119,9 @@ int lit_initialize(plb_state* s, double rho0, const double u0[3], const plb_init
 int lit_collide(plb_state* s, double tau, const double F[3]);
 int lit_push(plb_state* s);
 int lit_collide_push(plb_state* s, double tau, const double F[3]);
-int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc);
+// bc_vel: the faces whose velocity is reset; bc_rho: those whose density is
+int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc_vel,
+                             const plb_boundary* bc_rho);
 int lit_noslip(plb_state* s, const plb_boundary* bc);
 int lit_periodic(plb_state* s, const plb_boundary* bc);
 int lit_moments(plb_state* s);

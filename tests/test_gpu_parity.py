@@ -113,6 +113,40 @@ def test_literal_collide_then_push_bitexact(orc, q, X, Y, Z, prec):
             lit_compare(st, o, f"step {step} end")
 
 
+@pytest.mark.parametrize("q,X,Y,Z", LITERAL_CASES)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_literal_staged_steps_without_reads_bitexact(orc, q, X, Y, Z, prec):
+    """Whole steps of staged calls with nothing read in between, the call
+    sequence of PLBPD::march_in_time: the library folds each step's moments
+    and resets into the next collide+push kernel.  The resets get a boundary
+    of their own (another lid velocity and solid density than no-slip's), so
+    the fold must take the reset calls' arguments."""
+    print(f"\n[case] literal staged steps q={q} {X}x{Y}x{Z} {prec}", flush=True)
+    dt = np.float64 if prec == "f64" else np.float32
+    types = couette_types(q)
+    ob, pb = both_bc(types, {UP: (0.05, 0.01 if q != 9 else 0.0, 0.0), DOWN: (0.0, 0.0, 0.0)})
+    obr, pbr = both_bc(types, {UP: (0.02, 0.0, 0.004), DOWN: (0.0, 0.003, 0.0)}, rho_s=1.01)
+    tau, F = 0.7, (1e-4, -2e-5 if q != 9 else 0.0, 3e-5)
+    o = orc.lit_state(q, X, Y, Z, dt)
+    orc.lit_init(o, 1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 31, 1e-2), ob)
+    with P.State(q, X, Y, Z, prec, P.LITERAL) as st:
+        st.initialize(1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 31, 1e-2), pb)
+        for step in range(4):
+            orc.lit_collide(o, tau, F)
+            orc.lit_push(o)
+            assert orc.lit_noslip(o, ob) == 0
+            orc.lit_periodic(o, ob)
+            orc.lit_moments(o)
+            orc.lit_reset(o, obr)
+            st.collide(tau, F)
+            st.stream("push")
+            st.apply_noslip(pb)
+            st.apply_periodicity(pb)
+            st.calc_moments()
+            st.reset(pbr)
+        lit_compare(st, o, "after 4 staged steps")
+
+
 def test_literal_deferred_collision_runs_before_other_calls(orc):
     """A literal collision waits only for a directly following push.  Two
     collisions in a row both run; a stream request the literal mode refuses
