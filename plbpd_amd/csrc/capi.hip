@@ -460,13 +460,15 @@ int plb_stream(plb_state* s, const char* stream_type) {
     int rc = enter_keep(s);
     if (rc) return rc;
     const std::string t = stream_type ? stream_type : "";
-    if (t == "push" && lit_queue_is(s, {kLitMoments, kLitResetVelocity, kLitResetDensity, kLitCollide})) {
+    // a queue left from before the fusion knob was turned off runs call by call
+    const bool fold = s->tune_lit_fuse && t == "push";
+    if (fold && lit_queue_is(s, {kLitMoments, kLitResetVelocity, kLitResetDensity, kLitCollide})) {
         const std::vector<LitOp> q = std::move(s->lit_queue);
         s->lit_queue.clear();
         return timed(s, PLB_STAGE_COLLIDE,
                      [&] { return lit_moments_collide_push(s, q[3].tau, q[3].force, &q[1].bc, &q[2].bc); });
     }
-    if (t == "push" && lit_queue_is(s, {kLitCollide})) {
+    if (fold && lit_queue_is(s, {kLitCollide})) {
         const LitOp c = s->lit_queue[0];
         s->lit_queue.clear();
         return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, c.tau, c.force); });

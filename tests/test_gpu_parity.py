@@ -174,6 +174,13 @@ def test_literal_deferred_collision_runs_before_other_calls(orc):
         orc.lit_push(o)
         st.stream("push")
         lit_compare(st, o, "then push")
+        # the fusion knob turned off with a collision queued: it runs alone
+        orc.lit_collide(o, tau, F)
+        orc.lit_push(o)
+        st.collide(tau, F)
+        st.set_tuning(P.capi.TUNE_LITERAL_FUSE, 0)
+        st.stream("push")
+        lit_compare(st, o, "knob off with a queued collision")
 
 
 @pytest.mark.parametrize("q,X,Y,Z", LITERAL_CASES)
