@@ -176,7 +176,10 @@ int plb_set_boundary(plb_state* s, const plb_boundary* bc, const uint8_t* mask);
 int plb_collide_srt(plb_state* s, double tau, const double force[3]);
 /* Streaming::operator()(simdata, stream_type) (Streaming.cpp:31-42, 63-94):
  * "push" streams n into ntmp and swaps them; "pull" and anything else ->
- * PLB_ERR_LOGIC as in the reference (Streaming.cpp:17-23, 96-102).  Fused
+ * PLB_ERR_LOGIC as in the reference (Streaming.cpp:17-23, 96-102), after the
+ * literal calls deferred before it have run.  A literal "push" that follows
+ * a deferred collision runs it in the same kernel (with the deferred moments
+ * and resets of the step before, see plb_calc_moments).  Fused
  * mode: runs the fused kernel for "push" or "pull" (the pull streaming the
  * reference leaves unimplemented; both give the same numbers); "pull" while
  * PLB_TUNE_SCHEME forces push -> PLB_ERR_LOGIC. */
