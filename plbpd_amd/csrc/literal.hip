@@ -1,6 +1,9 @@
 // Literal mode: the reference's staged step (PLBPD.cpp:23-30), one kernel per
 // reference operator, on halo'd structure-of-arrays buffers:
 //   n[k * ncell + c],  c = (z*(Y+2) + y)*(X+2) + x,  x in [0, X+1] etc.
+// A collision and the push after it can run as one kernel (k_collide_push),
+// which can also take the previous step's moments and resets
+// (k_moments_collide_push); every buffer ends as after the separate kernels.
 // R = double (FP64 contract) or float (the reference's own rounding, with its
 // float/double promotions).  Compiled with -ffp-contract=off: every
 // expression below is evaluated in the order written, as in the reference.
