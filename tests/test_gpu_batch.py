@@ -83,6 +83,44 @@ def test_batch_moving_lid(orc, scheme, variant):
     b.close()
 
 
+@pytest.mark.parametrize("q", [9, 27])
+@pytest.mark.parametrize("variant", ["f64", "f32s"])
+@pytest.mark.parametrize("scheme", ["pull", "push"])
+@pytest.mark.parametrize("tile", [64, 256])
+def test_batch_other_lattices_and_ragged_domains(orc, q, variant, scheme, tile):
+    """D2Q9 and D3Q27 batches with domains that end mid-tile, one cell wide,
+    three layers thin (two walls and one fluid layer), and filling whole
+    tiles, next to each other in one pool."""
+    print(f"\n[case] batch q={q} {variant} {scheme} tile {tile}", flush=True)
+    if q == 9:
+        types = [PERIODIC, PERIODIC, 0, 0, NOSLIP, NOSLIP]
+        shapes = [(1, 1, 3), (64, 1, 4), (37, 1, 9), (5, 1, 3), (130, 1, 6)]
+    else:
+        types = [PERIODIC, PERIODIC, PERIODIC, PERIODIC, NOSLIP, NOSLIP]
+        shapes = [(1, 1, 3), (4, 4, 4), (7, 3, 5), (1, 5, 6), (16, 8, 3), (9, 11, 7)]
+    vel = {UP: (0.0, 0.0, 0.0), DOWN: (0.0, 0.0, 0.0)}
+    ob, pb = Boundary(type=types, vel=vel), P.boundary(types, vel)
+    rng = np.random.default_rng(q)
+    doms = [{"x": x, "y": y, "z": z, "tau": float(rng.uniform(0.6, 1.2)),
+             "force": (float(rng.uniform(1e-6, 1e-5)), 0.0, 0.0)} for x, y, z in shapes]
+    steps, seed, amp = 9, 70, 3e-3
+    b = P.Batch(q, variant, doms, pb)
+    b.set_scheme(2 if scheme == "pull" else 1)
+    b.set_tiling(tile, 0)
+    b.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, seed, amp))
+    b.step(steps)
+    b.synchronize()
+    for d, dm in enumerate(doms):
+        o = orc.cor_state(variant, q, dm["x"], dm["y"], dm["z"])
+        orc.cor_init(o, 1.0, (0, 0, 0), (P.INIT_PERTURBED, seed + d, amp))
+        orc.cor_run(o, dm["tau"], dm["force"], ob, steps)
+        orho, ou = orc.cor_moments(o, ob)
+        rho, u = b.moments(d)
+        assert np.array_equal(rho, orho), f"domain {d} {dm['x']}x{dm['y']}x{dm['z']} rho"
+        assert np.array_equal(u, ou), f"domain {d} {dm['x']}x{dm['y']}x{dm['z']} u"
+    b.close()
+
+
 def test_batch_autotune_and_cells():
     doms = domains(16, 9, 16, 40)
     _, pb = channel()
