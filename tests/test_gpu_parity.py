@@ -147,6 +147,46 @@ def test_literal_staged_steps_without_reads_bitexact(orc, q, X, Y, Z, prec):
         lit_compare(st, o, "after 4 staged steps")
 
 
+@pytest.mark.parametrize("q,X,Y,Z", [(9, 1, 1, 1), (9, 3, 1, 2), (19, 1, 1, 1), (19, 2, 1, 3), (27, 1, 1, 1),
+                                     (27, 1, 3, 2)])
+@pytest.mark.parametrize("staged", [False, True])
+def test_literal_tiny_domains_all_walls_bitexact(orc, q, X, Y, Z, staged):
+    """Domains where every cell lies on several faces, all six faces no-slip
+    with a velocity of their own: the face resets of each step (folded into
+    the next step's kernel) must leave the last face's velocity, as the
+    reference's face loop does.  Through plb_step and through the staged
+    calls."""
+    print(f"\n[case] literal tiny q={q} {X}x{Y}x{Z} staged={staged}", flush=True)
+    types = [NOSLIP] * 6
+    vel = {EAST: (0.0, 0.01, 0.0), WEST: (0.0, -0.01, 0.002), NORTH: (0.02, 0.0, 0.0),
+           SOUTH: (-0.01, 0.0, 0.003), UP: (0.03, 0.0, 0.0), DOWN: (0.0, 0.004, 0.0)}
+    ob, pb = both_bc(types, vel, rho_s=0.99)
+    tau, F = 0.75, (1e-4, 0.0, -2e-5)
+    o = orc.lit_state(q, X, Y, Z)
+    orc.lit_init(o, 1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 3, 1e-2), ob)
+    with P.State(q, X, Y, Z, "f64", P.LITERAL) as st:
+        st.initialize(1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 3, 1e-2), pb)
+        if staged:
+            for _ in range(5):
+                orc.lit_collide(o, tau, F)
+                orc.lit_push(o)
+                assert orc.lit_noslip(o, ob) == 0
+                orc.lit_periodic(o, ob)
+                orc.lit_moments(o)
+                orc.lit_reset(o, ob)
+                st.collide(tau, F)
+                st.stream("push")
+                st.apply_noslip(pb)
+                st.apply_periodicity(pb)
+                st.calc_moments()
+                st.reset(pb)
+        else:
+            orc.lit_run(o, tau, F, ob, 5)
+            st.configure(tau, F, pb)
+            st.step(5)
+        lit_compare(st, o, "after 5 steps")
+
+
 def test_literal_deferred_collision_runs_before_other_calls(orc):
     """A literal collision waits only for a directly following push.  Two
     collisions in a row both run; a stream request the literal mode refuses
