@@ -19,6 +19,9 @@ Reported:
                WRITE_SIZE, calibrated on a copy kernel) where a profile of
                this workload is in profiles/;
   variants     the same workload with FP64, FP32 and FP16-shifted storage;
+  literal      the reference-staged step (bit-exact FP64, and FP32 with the
+               reference's rounding) on the same lattice without the solids
+               the reference lacks, MLUPS;
   errors       velocity / density L2 error of each variant against the FP64
                run after --error-steps steps;
   cpu_baseline the reference's FP32 staged CPU step (restated with OpenMP and
@@ -105,6 +108,23 @@ def time_variant(P, variant, mask, steps, warmup, graph):
     scheme = st.scheme()
     st.close()
     return wall, ev_ms, scheme
+
+
+def time_literal(P, steps=20, warmup=2):
+    """MLUPS of the literal (reference-staged) step, FP64 and FP32, on the
+    headline lattice without interior solids (the reference has none)."""
+    out = {"config": f"D3Q19 {N}^3 channel, walls up/down, periodic x/y, tau {TAU}, g {FORCE[0]}; "
+                     f"plb_step in literal mode, {steps} timed steps"}
+    for prec in ("f64", "f32"):
+        bc = P.channel()
+        with P.State(Q, N, N, N, prec, P.LITERAL) as st:
+            st.initialize(1.0, (0.0, 0.0, 0.0), None, bc)
+            st.configure(TAU, FORCE, bc)
+            st.step(warmup)
+            st.synchronize()
+            ms = st.time_steps(steps) / steps
+        out[prec] = {"mlups": round(N ** 3 / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms, 4)}
+    return out
 
 
 def errors_vs_f64(P, variants, mask, steps):
@@ -283,6 +303,7 @@ def main():
         line["variants"] = var
         if "f64" in var:
             line["speedup_vs_f64"] = round(var[args.variant]["mlups"] / var["f64"]["mlups"], 3)
+        line["literal"] = time_literal(P)
 
     if not args.no_errors:
         print(f"[bench] errors vs FP64 after {args.error_steps} steps", file=sys.stderr, flush=True)
