@@ -1,8 +1,8 @@
 // C ABI of the plbpd HIP library: state lifecycle, one entry point per
 // reference operator call, fused stepping, HIP-graph replay, timers.
 #include <cmath>
-#include <initializer_list>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -82,8 +82,9 @@ int timer_end(plb_state* s, TimerPair* tp) {
 
 namespace {
 
-// Entry into a state's call without running a deferred literal collision:
-// only plb_stream uses it, to fuse that collision with its push.
+// Entry into a state's call that leaves the deferred literal calls queued:
+// plb_collide_srt and the resets may extend the queue, and plb_stream folds
+// it into its push.  Every other entry point uses enter(), which runs it.
 int enter_keep(plb_state* s) {
     if (!s) return fail(PLB_ERR_INVALID_ARGUMENT, "null state");
     PLB_HIP_TRY(hipSetDevice(s->device));
