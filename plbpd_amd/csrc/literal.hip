@@ -373,8 +373,9 @@ __device__ inline void apply_resets(const FaceResets<R>& fr, const LitDims& g, i
         if (fr.noslip[f] && co[f >> 1] == ((f & 1) == 0 ? dim[f >> 1] : 1)) rho = fr.rho_s;
 }
 
-// Inside the literal step loop: the previous step's moments and resets
-// (CalcMoments, Boundary::reset) folded into this step's collision and push.
+// The previous step's moments and resets (CalcMoments, Boundary::reset)
+// folded into this step's collision and push, inside plb_step or for staged
+// calls that the C ABI deferred until the push.
 // rho and u are computed from n exactly as k_moments and the resets leave
 // them, stored (no-slip reads rho before the next moments), and used by the
 // collision at once; n and t end as after k_collide_push.
