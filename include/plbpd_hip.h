@@ -276,6 +276,13 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                                 /* kernel (same rho and u), inside plb_step and for   */
                                 /* the staged calls alike; 0 runs one kernel per      */
                                 /* reference call                                      */
+#define PLB_TUNE_LITERAL_AVX2 9 /* literal FP32: 1 collides in the arithmetic of the */
+                                /* reference's AVX2 build (its default: FP32 fused   */
+                                /* multiply-adds for the first 8*floor(q/8) k,       */
+                                /* CollisionSRT.cpp:93-149); 0 (default) in that of  */
+                                /* its scalar kernel (CollisionSRT.cpp:153-176).  A   */
+                                /* collision uses the value set when it was called.  */
+                                /* No effect in FP64 or fused mode                    */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
  * step will use. */

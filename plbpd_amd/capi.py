@@ -30,7 +30,7 @@ LAYOUT_REFERENCE, LAYOUT_COMPACT = 0, 1
 DTYPE_F64, DTYPE_F32 = 0, 1
 INIT_UNIFORM, INIT_PERTURBED, INIT_TAYLOR_GREEN = 0, 1, 2
 (TUNE_BLOCK_X, TUNE_NT_LOAD, TUNE_NT_STORE, TUNE_SCHEME, TUNE_TILE_ORDER, TUNE_VECTOR, TUNE_XCD, TUNE_INDEX64,
- TUNE_LITERAL_FUSE) = range(9)
+ TUNE_LITERAL_FUSE, TUNE_LITERAL_AVX2) = range(10)
 SCHEME_AUTO, SCHEME_PUSH, SCHEME_PULL = 0, 1, 2
 STAGES = ["collide", "stream", "noslip", "periodicity", "moments", "velocity_reset",
           "density_reset", "fused"]

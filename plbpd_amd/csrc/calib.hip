@@ -51,6 +51,7 @@ int plb_set_tuning(plb_state* s, int key, int value) {
             break;
         case PLB_TUNE_INDEX64: s->tune_index64 = value != 0; break;
         case PLB_TUNE_LITERAL_FUSE: s->tune_lit_fuse = value != 0; break;
+        case PLB_TUNE_LITERAL_AVX2: s->tune_lit_avx2 = value != 0; break;
         case PLB_TUNE_VECTOR:
             if (value != -1 && value != 0 && value != 1 && value != 2 && value != 4)
                 return fail(PLB_ERR_INVALID_ARGUMENT, "row pull width must be -1, 0, 1, 2 or 4 cells per lane");

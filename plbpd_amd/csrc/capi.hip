@@ -123,7 +123,9 @@ int lit_flush(plb_state* s) {
             case kLitResetDensity:
                 rc = timed(s, PLB_STAGE_RESET_DENSITY, [&] { return lit_reset_density(s, &op.bc); });
                 break;
-            default: rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, op.tau, op.force); }); break;
+            default:
+                rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, op.tau, op.force, op.avx2 != 0); });
+                break;
         }
         if (rc) return rc;
     }
@@ -138,10 +140,12 @@ bool lit_queue_is(const plb_state* s, std::initializer_list<LitOpKind> kinds) {
     return true;
 }
 
-LitOp lit_op(LitOpKind kind, const plb_boundary* bc = nullptr, double tau = 0.0, const double* force = nullptr) {
+LitOp lit_op(LitOpKind kind, const plb_boundary* bc = nullptr, double tau = 0.0, const double* force = nullptr,
+             int avx2 = 0) {
     LitOp op;
     std::memset(&op, 0, sizeof(op));
     op.kind = kind;
+    op.avx2 = avx2;
     if (bc) op.bc = *bc;
     op.tau = tau;
     if (force)
@@ -284,6 +288,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->tune_xcd = -1;
     s->tune_index64 = 0;
     s->tune_lit_fuse = 1;
+    s->tune_lit_avx2 = 0;
     s->post_collision = false;
     for (double& v : s->stage_ms) v = 0.0;
 
@@ -445,8 +450,9 @@ int plb_collide_srt(plb_state* s, double tau, const double force[3]) {
             !(s->lit_queue.empty() || lit_queue_is(s, {kLitMoments, kLitResetVelocity, kLitResetDensity})))
             if ((rc = lit_flush(s))) return rc;
         if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
-        if (!s->tune_lit_fuse) return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, tau, force); });
-        s->lit_queue.push_back(lit_op(kLitCollide, nullptr, tau, force));
+        if (!s->tune_lit_fuse)
+            return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, tau, force, s->tune_lit_avx2 != 0); });
+        s->lit_queue.push_back(lit_op(kLitCollide, nullptr, tau, force, s->tune_lit_avx2));
         return PLB_OK;
     }
     if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
@@ -467,12 +473,15 @@ int plb_stream(plb_state* s, const char* stream_type) {
         const std::vector<LitOp> q = std::move(s->lit_queue);
         s->lit_queue.clear();
         return timed(s, PLB_STAGE_COLLIDE,
-                     [&] { return lit_moments_collide_push(s, q[3].tau, q[3].force, &q[1].bc, &q[2].bc); });
+                     [&] {
+                         return lit_moments_collide_push(s, q[3].tau, q[3].force, &q[1].bc, &q[2].bc,
+                                                         q[3].avx2 != 0);
+                     });
     }
     if (fold && lit_queue_is(s, {kLitCollide})) {
         const LitOp c = s->lit_queue[0];
         s->lit_queue.clear();
-        return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, c.tau, c.force); });
+        return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, c.tau, c.force, c.avx2 != 0); });
     }
     // every other request finds the deferred calls done, as the reference's does
     rc = lit_flush(s);
@@ -592,15 +601,16 @@ int plb_step(plb_state* s, int nsteps) {
             // last are computed by the next step's collision kernel, which
             // stores rho and u as those stages would before it collides.
             const plb_boundary* bc = &s->bc;
-            const bool fold = s->tune_lit_fuse != 0;
+            const bool fold = s->tune_lit_fuse != 0, avx2 = s->tune_lit_avx2 != 0;
             if (fold && i > 0) {
                 if ((rc = timed(s, PLB_STAGE_COLLIDE,
-                                [&] { return lit_moments_collide_push(s, s->tau, F, bc, bc); })))
+                                [&] { return lit_moments_collide_push(s, s->tau, F, bc, bc, avx2); })))
                     return rc;
             } else if (fold) {
-                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, s->tau, F); }))) return rc;
+                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, s->tau, F, avx2); })))
+                    return rc;
             } else {
-                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->tau, F); }))) return rc;
+                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->tau, F, avx2); }))) return rc;
                 if ((rc = timed(s, PLB_STAGE_STREAM, [&] { return lit_push(s); }))) return rc;
             }
             if ((rc = timed(s, PLB_STAGE_NOSLIP, [&] { return lit_noslip(s, bc); }))) return rc;

@@ -84,10 +84,23 @@ __global__ void k_init(LitDims g, R* n, R* ntmp, R* rho, R* u, double rho0, doub
 
 // CollisionSRT::_collision_kernel (CollisionSRT.cpp:153-176) for population
 // k of one cell, with _get_cu (84-91): the post-collision value of old.
-template <int Q, class R>
+// With AVX (FP32 only), the first 8 * floor(Q / 8) directions follow the
+// reference's AVX2 kernel instead (CollisionSRT.cpp:118-142, built by default,
+// src/CMakeLists.txt:48-51): single-precision fused multiply-adds in its order.
+template <int Q, class R, bool AVX = false>
 __device__ inline R collide_value(int k, R old, R rl, const R (&uu)[3], R usq, R om, R omm) {
     const R cu = (k == 0) ? (R)0
                           : (R)Lat<Q>::cx(k) * uu[0] + (R)Lat<Q>::cy(k) * uu[1] + (R)Lat<Q>::cz(k) * uu[2];
+    if (std::is_same<R, float>::value && AVX && k < 8 * (Q / 8)) {
+        const float kusq = -1.5f * (float)usq;
+        const float cusq = (float)cu * (float)cu;
+        float neq = __builtin_fmaf(3.0f, (float)cu, 1.0f);
+        neq = __builtin_fmaf(4.5f, cusq, neq);
+        neq = neq + kusq;
+        neq = neq * (float)rl;
+        neq = neq * (float)Lat<Q>::w(k);
+        return (R)__builtin_fmaf((float)omm, (float)old, (float)om * neq);
+    }
     if (std::is_same<R, float>::value) {
         const float wr = (float)Lat<Q>::w(k) * (float)rl;
         const double neq =
@@ -112,7 +125,7 @@ __device__ inline void updated_u(R u0, R u1, R u2, R rl, R tau, R F0, R F1, R F2
 }
 
 // CollisionSRT::operator() (CollisionSRT.cpp:25-176): in place, every interior cell.
-template <int Q, class R>
+template <int Q, class R, bool AVX>
 __global__ void k_collide(LitDims g, R* __restrict__ n, const R* __restrict__ rho,
                           const R* __restrict__ u, R tau, R om, R omm, R F0, R F1, R F2) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -125,7 +138,7 @@ __global__ void k_collide(LitDims g, R* __restrict__ n, const R* __restrict__ rh
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         R* p = n + k * g.ncell + c;
-        *p = collide_value<Q, R>(k, *p, rl, uu, usq, om, omm);
+        *p = collide_value<Q, R, AVX>(k, *p, rl, uu, usq, om, omm);
     }
 }
 
@@ -135,7 +148,7 @@ __global__ void k_collide(LitDims g, R* __restrict__ n, const R* __restrict__ rh
 // does not overwrite are read again), and at x + c_k in t, as k_push copies
 // it.  Both buffers end exactly as after k_collide then k_push; the push's
 // re-read of n is saved.
-template <int Q, class R>
+template <int Q, class R, bool AVX>
 __global__ void k_collide_push(LitDims g, R* __restrict__ n, const R* __restrict__ rho,
                                const R* __restrict__ u, R* __restrict__ t, R tau, R om, R omm, R F0, R F1,
                                R F2) {
@@ -148,7 +161,7 @@ __global__ void k_collide_push(LitDims g, R* __restrict__ n, const R* __restrict
     updated_u(u[c], u[g.ncell + c], u[2 * g.ncell + c], rl, tau, F0, F1, F2, uu, usq);
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        const R v = collide_value<Q, R>(k, n[k * g.ncell + c], rl, uu, usq, om, omm);
+        const R v = collide_value<Q, R, AVX>(k, n[k * g.ncell + c], rl, uu, usq, om, omm);
         n[k * g.ncell + c] = v;
         const int64_t off = (int64_t)Lat<Q>::cx(k) + (int64_t)Lat<Q>::cy(k) * g.nx +
                             (int64_t)Lat<Q>::cz(k) * g.nx * g.ny;
@@ -379,7 +392,7 @@ __device__ inline void apply_resets(const FaceResets<R>& fr, const LitDims& g, i
 // rho and u are computed from n exactly as k_moments and the resets leave
 // them, stored (no-slip reads rho before the next moments), and used by the
 // collision at once; n and t end as after k_collide_push.
-template <int Q, class R>
+template <int Q, class R, bool AVX>
 __global__ void k_moments_collide_push(LitDims g, R* __restrict__ n, R* __restrict__ rho, R* __restrict__ u,
                                        R* __restrict__ t, FaceResets<R> fr, R tau, R om, R omm, R F0, R F1,
                                        R F2) {
@@ -405,7 +418,7 @@ __global__ void k_moments_collide_push(LitDims g, R* __restrict__ n, R* __restri
     updated_u(um[0], um[1], um[2], rl, tau, F0, F1, F2, uu, usq);
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        const R v = collide_value<Q, R>(k, nv[k], rl, uu, usq, om, omm);
+        const R v = collide_value<Q, R, AVX>(k, nv[k], rl, uu, usq, om, omm);
         n[k * g.ncell + c] = v;
         const int64_t off = (int64_t)Lat<Q>::cx(k) + (int64_t)Lat<Q>::cy(k) * g.nx +
                             (int64_t)Lat<Q>::cz(k) * g.nx * g.ny;
@@ -523,31 +536,41 @@ void relaxation(double tau, R* tr, R* om, R* omm) {
 }
 }  // namespace
 
-int lit_collide(plb_state* s, double tau, const double F[3]) {
+int lit_collide(plb_state* s, double tau, const double F[3], bool avx2) {
     const LitDims g = dims_of(s);
     return with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
         using R = decltype(rr);
         R tr, om, omm;
         relaxation(tau, &tr, &om, &omm);
-        hipLaunchKernelGGL((k_collide<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
-                           (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, tr, om, omm,
-                           (R)F[0], (R)F[1], (R)F[2]);
+        if (avx2 && std::is_same<R, float>::value)
+            hipLaunchKernelGGL((k_collide<Q, R, true>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
+                               (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, tr, om, omm, (R)F[0], (R)F[1],
+                               (R)F[2]);
+        else
+            hipLaunchKernelGGL((k_collide<Q, R, false>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
+                               (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, tr, om, omm, (R)F[0], (R)F[1],
+                               (R)F[2]);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
 }
 
-int lit_collide_push(plb_state* s, double tau, const double F[3]) {
+int lit_collide_push(plb_state* s, double tau, const double F[3], bool avx2) {
     const LitDims g = dims_of(s);
     const int rc = with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
         using R = decltype(rr);
         R tr, om, omm;
         relaxation(tau, &tr, &om, &omm);
-        hipLaunchKernelGGL((k_collide_push<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
-                           (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, (R*)s->buf[s->cur ^ 1], tr, om,
-                           omm, (R)F[0], (R)F[1], (R)F[2]);
+        if (avx2 && std::is_same<R, float>::value)
+            hipLaunchKernelGGL((k_collide_push<Q, R, true>), interior_grid(s), dim3(interior_block(s)), 0, s->stream,
+                               g, (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, (R*)s->buf[s->cur ^ 1], tr,
+                               om, omm, (R)F[0], (R)F[1], (R)F[2]);
+        else
+            hipLaunchKernelGGL((k_collide_push<Q, R, false>), interior_grid(s), dim3(interior_block(s)), 0,
+                               s->stream, g, (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u,
+                               (R*)s->buf[s->cur ^ 1], tr, om, omm, (R)F[0], (R)F[1], (R)F[2]);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
@@ -556,7 +579,7 @@ int lit_collide_push(plb_state* s, double tau, const double F[3]) {
 }
 
 int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc_vel,
-                             const plb_boundary* bc_rho) {
+                             const plb_boundary* bc_rho, bool avx2) {
     const LitDims g = dims_of(s);
     const int rc = with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
@@ -570,9 +593,14 @@ int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const 
             for (int i = 0; i < 3; ++i) fr.vel[f][i] = (R)bc_vel->vel[f][i];
         }
         fr.rho_s = (R)bc_rho->rho_s;
-        hipLaunchKernelGGL((k_moments_collide_push<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream,
-                           g, (R*)s->buf[s->cur], (R*)s->rho, (R*)s->u, (R*)s->buf[s->cur ^ 1], fr, tr, om, omm,
-                           (R)F[0], (R)F[1], (R)F[2]);
+        if (avx2 && std::is_same<R, float>::value)
+            hipLaunchKernelGGL((k_moments_collide_push<Q, R, true>), interior_grid(s), dim3(interior_block(s)), 0,
+                               s->stream, g, (R*)s->buf[s->cur], (R*)s->rho, (R*)s->u, (R*)s->buf[s->cur ^ 1], fr,
+                               tr, om, omm, (R)F[0], (R)F[1], (R)F[2]);
+        else
+            hipLaunchKernelGGL((k_moments_collide_push<Q, R, false>), interior_grid(s), dim3(interior_block(s)), 0,
+                               s->stream, g, (R*)s->buf[s->cur], (R*)s->rho, (R*)s->u, (R*)s->buf[s->cur ^ 1], fr,
+                               tr, om, omm, (R)F[0], (R)F[1], (R)F[2]);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });

@@ -53,6 +53,7 @@ struct LitOp {
     LitOpKind kind;
     double tau;        // kLitCollide
     double force[3];   // kLitCollide
+    int avx2;          // kLitCollide: PLB_TUNE_LITERAL_AVX2 at the call
     plb_boundary bc;   // the resets
 };
 
@@ -99,6 +100,7 @@ struct plb_state {
     int tune_index64;     // 1: 64-bit index kernels even where 32-bit offsets fit
     int tune_vector;      // row pull kernel cells per lane (1, 2, 4) where X allows; 0 off; -1 auto
     int tune_lit_fuse;    // literal: 1 collision and push in one kernel when they follow each other
+    int tune_lit_avx2;    // literal FP32: 1 collision arithmetic of the reference's AVX2 kernel
     bool post_collision;  // fused: buf[cur] holds post-collision populations (pull step)
     // timers
     bool timers_on;
@@ -116,12 +118,13 @@ int timer_end(plb_state* s, TimerPair* tp);
 // literal mode (literal.hip)
 int lit_initialize(plb_state* s, double rho0, const double u0[3], const plb_init_spec* init,
                    const double* tg_dev);
-int lit_collide(plb_state* s, double tau, const double F[3]);
+// avx2: FP32 collision in the arithmetic of the reference's AVX2 kernel
+int lit_collide(plb_state* s, double tau, const double F[3], bool avx2);
 int lit_push(plb_state* s);
-int lit_collide_push(plb_state* s, double tau, const double F[3]);
+int lit_collide_push(plb_state* s, double tau, const double F[3], bool avx2);
 // bc_vel: the faces whose velocity is reset; bc_rho: those whose density is
 int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc_vel,
-                             const plb_boundary* bc_rho);
+                             const plb_boundary* bc_rho, bool avx2);
 int lit_noslip(plb_state* s, const plb_boundary* bc);
 int lit_periodic(plb_state* s, const plb_boundary* bc);
 int lit_moments(plb_state* s);

@@ -187,6 +187,51 @@ def test_literal_tiny_domains_all_walls_bitexact(orc, q, X, Y, Z, staged):
         lit_compare(st, o, "after 5 steps")
 
 
+@pytest.mark.parametrize("q,X,Y,Z", LITERAL_CASES)
+@pytest.mark.parametrize("path", ["step", "staged_reads", "staged"])
+def test_literal_fp32_avx2_arithmetic_bitexact(orc, q, X, Y, Z, path):
+    """FP32 literal mode with PLB_TUNE_LITERAL_AVX2: the collision in the
+    arithmetic of the reference's default AVX2 build (fused multiply-adds for
+    the first 8 * floor(q / 8) directions), against the oracle's restatement
+    of it, which tests/test_oracle_avx2.py pins.  Through plb_step, through
+    the staged calls with a read after every stage, and through staged calls
+    the library folds."""
+    print(f"\n[case] literal f32 avx2 q={q} {X}x{Y}x{Z} {path}", flush=True)
+    types = couette_types(q)
+    ob, pb = both_bc(types, {UP: (0.04, 0.0, 0.0), DOWN: (0.0, 0.0, 0.0)})
+    tau, F = 0.62, (2e-4, -1e-5 if q != 9 else 0.0, 1e-5)
+    o = orc.lit_state(q, X, Y, Z, np.float32)
+    orc.lit_init(o, 1.0, (0.02, 0.0, 0.0), (P.INIT_PERTURBED, 17, 1e-2), ob)
+    orc.lit_set_avx2(True)
+    try:
+        with P.State(q, X, Y, Z, "f32", P.LITERAL) as st:
+            st.set_tuning(P.capi.TUNE_LITERAL_AVX2, 1)
+            st.initialize(1.0, (0.02, 0.0, 0.0), (P.INIT_PERTURBED, 17, 1e-2), pb)
+            if path == "step":
+                orc.lit_run(o, tau, F, ob, 12)
+                st.configure(tau, F, pb)
+                st.step(12)
+            else:
+                for step in range(4):
+                    orc.lit_collide(o, tau, F)
+                    st.collide(tau, F)
+                    if path == "staged_reads":
+                        lit_compare(st, o, f"step {step} collide")
+                    orc.lit_push(o)
+                    assert orc.lit_noslip(o, ob) == 0
+                    orc.lit_periodic(o, ob)
+                    orc.lit_moments(o)
+                    orc.lit_reset(o, ob)
+                    st.stream("push")
+                    st.apply_noslip(pb)
+                    st.apply_periodicity(pb)
+                    st.calc_moments()
+                    st.reset(pb)
+            lit_compare(st, o, f"{path} end")
+    finally:
+        orc.lit_set_avx2(False)
+
+
 def test_literal_deferred_collision_runs_before_other_calls(orc):
     """A literal collision waits only for a directly following push.  Two
     collisions in a row both run; a stream request the literal mode refuses
