@@ -41,6 +41,11 @@ int orc_lit_init_f64(int q, int X, int Y, int Z, double rho0, const double* u0, 
                      const orc_bc* bc, double* rho, double* u, double* n, double* ntmp);
 int orc_lit_run_f64(int q, int X, int Y, int Z, double tau, const double* F, const orc_bc* bc, int nsteps,
                     double* rho, double* u, double* bufA, double* bufB);
+int orc_lit_init_f32(int q, int X, int Y, int Z, double rho0, const double* u0, const orc_init* in,
+                     const orc_bc* bc, float* rho, float* u, float* n, float* ntmp);
+int orc_lit_run_f32(int q, int X, int Y, int Z, double tau, const double* F, const orc_bc* bc, int nsteps,
+                    float* rho, float* u, float* bufA, float* bufB);
+void orc_lit_set_avx2(int on);
 int orc_cor_init(int variant, int q, int X, int Y, int Z, double rho0, const double* u0, const orc_init* in,
                  void* f);
 int orc_cor_run(int variant, int q, int X, int Y, int Z, double tau, const double* F, const orc_bc* bc,
@@ -303,6 +308,42 @@ void test_literal_march_matches_oracle() {
     EXPECT(std::memcmp(n.data(), sim.n->data(), 19 * cells * sizeof(double)) == 0);
 }
 
+// An FP32 literal state marched through the drop-in classes: CollisionSRT is
+// called with reference = false, the reference's default loop, so the
+// collision follows the AVX2 kernel of its default build.
+void test_literal_fp32_march_matches_avx2_oracle() {
+    const int X = 12, Y = 8, Z = 10, steps = 30;
+    const double tau = 0.7;
+    const std::array<double, 3> F = {1e-5, 0.0, 0.0};
+    Setup s = couette(X, Y, Z, tau, F, 0.05f);
+    plbpd::DeviceConfig cfg;
+    cfg.mode = PLB_MODE_LITERAL;
+    cfg.variant = PLB_F32;
+    SimData sim(s.domain->get_dimensions(), 19, cfg);
+    PLBPD p(s.model.get(), s.domain.get(), s.boundary.get());
+    p.initialize(sim);
+    p.march_in_time(steps, sim);
+    sim.sync_to_host();
+
+    const size_t cells = (size_t)(X + 2) * (Y + 2) * (Z + 2);
+    std::vector<float> rho(cells), u(3 * cells), a(19 * cells), b(19 * cells);
+    const orc_bc ob = to_orc(s.boundary->descriptor());
+    const orc_init in = {PLB_INIT_PERTURBED, 77, 1e-3};
+    const double u0[3] = {0, 0, 0};
+    orc_lit_init_f32(19, X, Y, Z, 1.0, u0, &in, &ob, rho.data(), u.data(), a.data(), b.data());
+    orc_lit_set_avx2(1);
+    const int which = orc_lit_run_f32(19, X, Y, Z, tau, F.data(), &ob, steps, rho.data(), u.data(), a.data(),
+                                      b.data());
+    orc_lit_set_avx2(0);
+    const std::vector<float>& n = which == 0 ? a : b;
+    bool same = true;
+    for (size_t i = 0; i < cells; ++i) same &= (double)rho[i] == (double)sim.rho->data()[i];
+    for (size_t i = 0; i < 3 * cells; ++i) same &= (double)u[i] == (double)sim.u->data()[i];
+    for (size_t i = 0; i < 19 * cells; ++i) same &= (double)n[i] == (double)sim.n->data()[i];
+    if (!same) std::fprintf(stderr, "FP32 literal march differs from the AVX2-arithmetic oracle\n");
+    EXPECT(same);
+}
+
 void test_fused_march_matches_oracle(int variant, float lid) {
     const int X = 16, Y = 8, Z = 10, steps = 30;
     const double tau = 0.65;
@@ -495,6 +536,7 @@ int main(int argc, char** argv) {
     }
     if (what == "gpu" || what == "all") {
         test_literal_march_matches_oracle();
+        test_literal_fp32_march_matches_avx2_oracle();
         for (int v : {PLB_F64, PLB_F32S, PLB_F16S}) test_fused_march_matches_oracle(v, 0.0f);
         for (int v : {PLB_F64, PLB_F32S}) test_fused_march_matches_oracle(v, 0.03f);  // moving lid
         test_scenario_march_matches_oracle();
