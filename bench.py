@@ -159,6 +159,27 @@ def errors_vs_f64(P, variants, mask, steps):
     return out
 
 
+def pattern_ceiling(capi, variant, solids, achieved):
+    """The step's memory pattern without its arithmetic (plb_bench_pattern),
+    each rate counted in the step's algorithmic bytes: a Q-stream copy, the
+    row kernel's pulls, and its pulls around interior solids (own reverse
+    rows and neighbour masks as well).  The kernel's own pattern is the last
+    with solids, the second without."""
+    sb = STORAGE_BYTES[variant]
+    # the widths the step's kernel uses: 4 cells per lane around solids for
+    # 4-byte storage, 2 for 8-byte, 1 without solids and for 2-byte storage
+    vec = {4: 4, 8: 2}.get(sb, 1) if solids else 1
+    out = {"unit": "GB/s", "cells_per_lane": vec, "storage_bytes": sb}
+    for name, pat in (("q_stream_copy", capi.PATTERN_COPY), ("pull", capi.PATTERN_PULL),
+                      ("pull_bounce_back", capi.PATTERN_PULL_SOLIDS)):
+        g, _ = capi.bench_pattern(Q, N, N, N, sb, vec, pat, iters=20)
+        out[name] = round(g, 1)
+        out[name + "_frac"] = round(g / HBM_PEAK_GBS, 4)
+    out["kernel_pattern"] = "pull_bounce_back" if solids else "pull"
+    out["kernel_frac_of_pattern"] = round(achieved / out[out["kernel_pattern"]], 4)
+    return out
+
+
 def cpu_baseline(sample_steps=150, single_core_steps=8):
     """About 10 s of work on all host cores and 7 s on one (at about 250 and
     20 MLUPS): a bounded sample of the same lattice, reported, not targeted."""
@@ -281,6 +302,7 @@ def main():
     gbs, _ = capi.bench_copy(1 << 32, 20)
     line["copy_gbps"] = round(gbs, 1)
     line["roofline"]["frac_of_copy"] = round(achieved / gbs, 4)
+    line["roofline"]["pattern_ceiling"] = pattern_ceiling(capi, args.variant, mask is not None, achieved)
     try:
         line["device"] = P.device_info(0)
     except Exception:

@@ -234,6 +234,18 @@ int plb_time_steps(plb_state* s, int nsteps, int steps_per_graph, double* ms);
 /* Copy-kernel calibration: copies `bytes` device-to-device `iters` times;
  * gbps counts bytes read plus bytes written. */
 int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms_per_copy);
+/* Memory-pattern ceiling of the fused step: the loads and stores of one
+ * step over a q-population x*y*z lattice (q 19 or 27) in the fused layout,
+ * with sbytes-wide raw values (2, 4, 8) and vec consecutive cells per lane
+ * (1, 2, 4; x a multiple of vec), without the collision.  pattern 0: a
+ * q-stream copy (every population read and written at its own cell);
+ * 1: the row pull kernel's loads (population k from row y - c_y, z - c_z)
+ * and flag bytes; 2: pattern 1 with the kernel's loads around interior
+ * solids (the cell's own reverse slots and a 4-byte neighbour mask).  gbps
+ * counts the step's algorithmic bytes, x*y*z*(2*q*sbytes + 1), over the
+ * mean launch time. */
+int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int vec, int pattern, int iters,
+                      double* gbps, double* ms);
 /* Fused-kernel tuning knobs (no reference counterpart).  The defaults are
  * the settings measured fastest on MI355X for D3Q19 256^3; no knob changes
  * results; a change drops a captured step graph. */
@@ -276,13 +288,6 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
                                 /* kernel (same rho and u), inside plb_step and for   */
                                 /* the staged calls alike; 0 runs one kernel per      */
                                 /* reference call                                      */
-#define PLB_TUNE_LITERAL_AVX2 9 /* literal FP32: 1 collides in the arithmetic of the */
-                                /* reference's AVX2 build (its default: FP32 fused   */
-                                /* multiply-adds for the first 8*floor(q/8) k,       */
-                                /* CollisionSRT.cpp:93-149); 0 (default) in that of  */
-                                /* its scalar kernel (CollisionSRT.cpp:153-176).  A   */
-                                /* collision uses the value set when it was called.  */
-                                /* No effect in FP64 or fused mode                    */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
  * step will use. */
@@ -319,7 +324,11 @@ int plb_batch_set_mapping(plb_batch* b, int xcd_runs);
 int plb_batch_get_mapping(const plb_batch* b, int* xcd_runs);
 /* Times each candidate tiling and mapping over `steps` steps and keeps the
  * fastest.  results: up to max_results rows of {tile_cells, grid_blocks,
- * mapping, ms_per_step}. */
+ * mapping, ms_per_step}; *nresults = the rows written.  The candidates run
+ * real steps on the batch: every candidate (60: 6 tile sizes x 5 grids x 2
+ * mappings) advances all domains by 2 + (steps rounded up to even) steps, so
+ * the call leaves the simulation that many steps times 60 further on (an
+ * even count); initialise the batch again to start from its initial state. */
 int plb_batch_autotune(plb_batch* b, int steps, double* results, int max_results, int* nresults);
 /* nsteps of all domains: one launch per step (graph-captured when
  * steps_per_graph > 0). */

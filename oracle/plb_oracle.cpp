@@ -296,17 +296,10 @@ void lit_init(const Lit<R>& g, double rho0, const double u0[3], const orc_init* 
             }
 }
 
-// FP32 collision arithmetic of the reference's AVX2 build (on by default,
-// src/CMakeLists.txt:48-51; CollisionSRT.cpp:93-149) instead of the scalar
-// kernel: set by orc_lit_set_avx2.
-int g_lit_avx2 = 0;
-
 // CollisionSRT::_collision_kernel (CollisionSRT.cpp:153-176) with
 // _get_updated_u (72-81) and _get_cu (84-91), every interior cell.  In FP32
-// with g_lit_avx2, the first 8 * floor(q / 8) directions follow
-// _collision_kernel_avx2 (CollisionSRT.cpp:118-142): single-precision fused
-// multiply-adds in its order; the rest is its scalar tail (144-147), which is
-// the scalar kernel.
+// the reference's promotions: w_k * rho in float, the bracket in double, the
+// blend's first product in float and its sum in double.
 template <class R>
 void lit_collide(const Lit<R>& g, double tau, const double F[3], const R* rho, const R* u, R* n) {
     const int q = g.L->q;
@@ -337,16 +330,7 @@ void lit_collide(const Lit<R>& g, double tau, const double F[3], const R* rho, c
                     const R cu = (k == 0) ? (R)0
                                           : (R)L.c[k][0] * uu[0] + (R)L.c[k][1] * uu[1] +
                                                 (R)L.c[k][2] * uu[2];
-                    if (std::is_same<R, float>::value && g_lit_avx2 && k < 8 * (q / 8)) {
-                        const float kusq = -1.5f * (float)usq;
-                        const float cusq = (float)cu * (float)cu;
-                        float neq = std::fmaf(3.0f, (float)cu, 1.0f);
-                        neq = std::fmaf(4.5f, cusq, neq);
-                        neq = neq + kusq;
-                        neq = neq * (float)rl;
-                        neq = neq * (float)L.w[k];
-                        nl[k] = (R)std::fmaf((float)omm, (float)nl[k], (float)om * neq);
-                    } else if (std::is_same<R, float>::value) {
+                    if (std::is_same<R, float>::value) {
                         const float wr = (float)L.w[k] * (float)rl;
                         const double neq = (double)wr * (1.0 + 3.0 * (double)cu +
                                                          4.5 * (double)cu * (double)cu -
@@ -817,10 +801,6 @@ size_t storage_bytes(int variant) {
 
 // =================================================================== C ABI ==
 extern "C" {
-
-// 1: the FP32 literal collision follows the reference's AVX2 kernel; 0 (the
-// default): its scalar kernel.
-void orc_lit_set_avx2(int on) { g_lit_avx2 = on != 0; }
 
 int orc_num_threads(void) {
 #ifdef _OPENMP

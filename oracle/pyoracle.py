@@ -90,7 +90,6 @@ class Oracle:
         L.orc_unit_uniform.restype = d
         L.orc_set_num_threads.argtypes = [i]
         L.orc_num_threads.restype = i
-        L.orc_lit_set_avx2.argtypes = [i]
         for suf in ("f64", "f32"):
             getattr(L, f"orc_lit_init_{suf}").argtypes = [i, i, i, i, d, vp, vp, vp, vp, vp, vp, vp]
             getattr(L, f"orc_lit_collide_{suf}").argtypes = [i, i, i, i, d, vp, vp, vp, vp]
@@ -162,11 +161,6 @@ class Oracle:
 
     def _dims(self, st):
         return st["q"], st["X"], st["Y"], st["Z"]
-
-    def lit_set_avx2(self, on):
-        """FP32 literal collisions follow the reference's AVX2 kernel (True)
-        or its scalar kernel (False, the default); process-wide."""
-        self.lib.orc_lit_set_avx2(1 if on else 0)
 
     def lit_collide(self, st, tau, F):
         fn = getattr(self.lib, f"orc_lit_collide_{st['suf']}")

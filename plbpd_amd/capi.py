@@ -30,7 +30,7 @@ LAYOUT_REFERENCE, LAYOUT_COMPACT = 0, 1
 DTYPE_F64, DTYPE_F32 = 0, 1
 INIT_UNIFORM, INIT_PERTURBED, INIT_TAYLOR_GREEN = 0, 1, 2
 (TUNE_BLOCK_X, TUNE_NT_LOAD, TUNE_NT_STORE, TUNE_SCHEME, TUNE_TILE_ORDER, TUNE_VECTOR, TUNE_XCD, TUNE_INDEX64,
- TUNE_LITERAL_FUSE, TUNE_LITERAL_AVX2) = range(10)
+ TUNE_LITERAL_FUSE) = range(9)
 SCHEME_AUTO, SCHEME_PUSH, SCHEME_PULL = 0, 1, 2
 STAGES = ["collide", "stream", "noslip", "periodicity", "moments", "velocity_reset",
           "density_reset", "fused"]
@@ -151,6 +151,7 @@ def lib():
     L.plb_stage_times.argtypes = [vp, vp]
     L.plb_time_steps.argtypes = [vp, i, i, C.POINTER(C.c_double)]
     L.plb_bench_copy.argtypes = [i, sz, i, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.plb_bench_pattern.argtypes = [i] * 9 + [C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.plb_set_tuning.argtypes = [vp, i, i]
     L.plb_step_scheme.argtypes = [vp, C.POINTER(C.c_int)]
     if hasattr(L, "plb_batch_create"):
@@ -200,6 +201,17 @@ def bench_copy(nbytes=1 << 31, iters=20, device=0):
     """Device copy bandwidth (GB/s, read + write) and ms per copy."""
     g, ms = C.c_double(), C.c_double()
     check(lib().plb_bench_copy(device, int(nbytes), int(iters), C.byref(g), C.byref(ms)))
+    return g.value, ms.value
+
+
+PATTERN_COPY, PATTERN_PULL, PATTERN_PULL_SOLIDS = 0, 1, 2
+
+
+def bench_pattern(q, X, Y, Z, sbytes, vec, pattern, iters=20, device=0):
+    """Memory-pattern ceiling of the fused step (plb_bench_pattern): the
+    step's algorithmic GB/s at the pattern's mean launch time, and ms."""
+    g, ms = C.c_double(), C.c_double()
+    check(lib().plb_bench_pattern(device, q, X, Y, Z, sbytes, vec, pattern, int(iters), C.byref(g), C.byref(ms)))
     return g.value, ms.value
 
 

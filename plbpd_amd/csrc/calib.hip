@@ -1,7 +1,9 @@
 // Measurement helpers: a copy kernel that calibrates the achievable HBM
-// bandwidth, event-timed stepping for throughput and roofline figures, and
-// the fused kernel's tuning knobs.
+// bandwidth, the memory-pattern kernels that bound the fused step, event-
+// timed stepping for throughput and roofline figures, and the fused kernel's
+// tuning knobs.
 #include <string>
+#include <type_traits>
 
 #include "state.hpp"
 
@@ -16,6 +18,124 @@ __global__ __launch_bounds__(256) void k_copy16(const u32x4* __restrict__ src, u
                                                 uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+template <class S, int VEC, bool NT>
+__device__ inline void pat_load(const S* p, S (&v)[VEC]) {
+    typedef S SV __attribute__((ext_vector_type(VEC)));
+    if constexpr (VEC == 1) {
+        v[0] = NT ? __builtin_nontemporal_load(p) : *p;
+    } else {
+        const SV w = NT ? __builtin_nontemporal_load(reinterpret_cast<const SV*>(p)) : *reinterpret_cast<const SV*>(p);
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) v[j] = w[j];
+    }
+}
+
+template <class S, int VEC>
+__device__ inline void pat_store(S* p, const S (&v)[VEC]) {
+    typedef S SV __attribute__((ext_vector_type(VEC)));
+    if constexpr (VEC == 1) {
+        __builtin_nontemporal_store(v[0], p);
+    } else {
+        SV w;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) w[j] = v[j];
+        __builtin_nontemporal_store(w, reinterpret_cast<SV*>(p));
+    }
+}
+
+// The loads and stores of one fused pull step without its arithmetic, in the
+// fused layout (population k of cell c at k * ps + c, x fastest), VEC
+// consecutive cells per lane, raw bits (S of the storage width):
+//   PAT 0  every population read and written at its own cell: a Q-stream
+//          copy, non-temporal both ways;
+//   PAT 1  the row kernel's pulls: population k read from row (y - c_y,
+//          z - c_z) at the same x (the x shift is a register move), the
+//          cell's flag byte, non-temporal pulls and stores (its pattern
+//          without interior solids);
+//   PAT 2  PAT 1 with the row kernel's pattern around interior solids:
+//          ordinary pulls, the cell's own reverse slots of every population
+//          and a 4-byte neighbour mask per cell, selected by the mask.
+// Every index is in range: c0 + VEC <= ncell (ncell is a multiple of X, X of
+// VEC), rows wrap into [0, Y) and [0, Z), k * ps + c < Q * ps.
+template <int Q, class S, int VEC, int PAT>
+__global__ __launch_bounds__(256) void k_pattern(const S* __restrict__ A, S* __restrict__ B,
+                                                 const uint8_t* __restrict__ flags,
+                                                 const uint32_t* __restrict__ nbmask, int X, int Y, int Z,
+                                                 uint64_t ps, uint64_t ncell) {
+    const uint64_t c0 = ((uint64_t)blockIdx.x * 256u + threadIdx.x) * VEC;
+    if (c0 >= ncell) return;
+    const uint64_t row = c0 / (uint64_t)X;
+    const int x0 = (int)(c0 - row * (uint64_t)X);
+    const int y = (int)(row % (uint64_t)Y), z = (int)(row / (uint64_t)Y);
+    S v[Q][VEC];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        uint64_t src = c0;
+        if (PAT > 0) {
+            int yy = y - Lat<Q>::cy(k), zz = z - Lat<Q>::cz(k);
+            yy = yy < 0 ? yy + Y : (yy >= Y ? yy - Y : yy);
+            zz = zz < 0 ? zz + Z : (zz >= Z ? zz - Z : zz);
+            src = ((uint64_t)zz * Y + yy) * X + x0;
+        }
+        pat_load<S, VEC, PAT != 2>(A + k * ps + src, v[k]);
+    }
+    if (PAT == 2) {
+        S own[Q][VEC];
+#pragma unroll
+        for (int k = 1; k < Q; ++k) pat_load<S, VEC, false>(A + rev<Q>(k) * ps + c0, own[k]);
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+            const uint32_t m = nbmask[c0 + j];
+#pragma unroll
+            for (int k = 1; k < Q; ++k)
+                if ((m >> rev<Q>(k)) & 1u) v[k][j] = own[k][j];
+        }
+    } else if (PAT == 1) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+            if (flags[c0 + j])
+#pragma unroll
+                for (int k = 0; k < Q; ++k) v[k][j] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) pat_store<S, VEC>(B + k * ps + c0, v[k]);
+}
+
+template <int Q, class S, int VEC, int PAT>
+void launch_pattern(const void* A, void* B, const uint8_t* flags, const uint32_t* mask, int X, int Y, int Z,
+                    uint64_t ps, uint64_t ncell, hipStream_t st) {
+    const unsigned nb = (unsigned)((ncell / VEC + 255) / 256);
+    hipLaunchKernelGGL((k_pattern<Q, S, VEC, PAT>), dim3(nb), dim3(256), 0, st, (const S*)A, (S*)B, flags, mask, X,
+                       Y, Z, ps, ncell);
+}
+
+// The launch for (q, storage bytes, cells per lane, pattern), or false.
+bool pattern_launch(int q, int sbytes, int vec, int pat, const void* A, void* B, const uint8_t* flags,
+                    const uint32_t* mask, int X, int Y, int Z, uint64_t ps, uint64_t ncell, hipStream_t st) {
+    bool ok = false;
+    auto by_pat = [&](auto qc, auto sc, auto vc) {
+        constexpr int Qv = decltype(qc)::value, V = decltype(vc)::value;
+        using S = decltype(sc);
+        if (pat == 0) launch_pattern<Qv, S, V, 0>(A, B, flags, mask, X, Y, Z, ps, ncell, st);
+        if (pat == 1) launch_pattern<Qv, S, V, 1>(A, B, flags, mask, X, Y, Z, ps, ncell, st);
+        if (pat == 2) launch_pattern<Qv, S, V, 2>(A, B, flags, mask, X, Y, Z, ps, ncell, st);
+        ok = pat >= 0 && pat <= 2;
+    };
+    auto by_vec = [&](auto qc, auto sc) {
+        if (vec == 1) by_pat(qc, sc, std::integral_constant<int, 1>());
+        if (vec == 2) by_pat(qc, sc, std::integral_constant<int, 2>());
+        if (vec == 4) by_pat(qc, sc, std::integral_constant<int, 4>());
+    };
+    auto by_size = [&](auto qc) {
+        if (sbytes == 2) by_vec(qc, uint16_t());
+        if (sbytes == 4) by_vec(qc, uint32_t());
+        if (sbytes == 8) by_vec(qc, uint64_t());
+    };
+    if (q == 19) by_size(std::integral_constant<int, 19>());
+    if (q == 27) by_size(std::integral_constant<int, 27>());
+    return ok;
 }
 
 }  // namespace
@@ -51,7 +171,6 @@ int plb_set_tuning(plb_state* s, int key, int value) {
             break;
         case PLB_TUNE_INDEX64: s->tune_index64 = value != 0; break;
         case PLB_TUNE_LITERAL_FUSE: s->tune_lit_fuse = value != 0; break;
-        case PLB_TUNE_LITERAL_AVX2: s->tune_lit_avx2 = value != 0; break;
         case PLB_TUNE_VECTOR:
             if (value != -1 && value != 0 && value != 1 && value != 2 && value != 4)
                 return fail(PLB_ERR_INVALID_ARGUMENT, "row pull width must be -1, 0, 1, 2 or 4 cells per lane");
@@ -141,6 +260,65 @@ int plb_bench_copy(int device, size_t bytes, int iters, double* gbps, double* ms
     const double per = (double)t / iters;
     *gbps = 2.0 * (double)(elems * 16) / (per * 1e-3) / 1e9;
     if (ms_per_copy) *ms_per_copy = per;
+    return PLB_OK;
+}
+
+int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int vec, int pattern, int iters,
+                      double* gbps, double* ms) {
+    if (!gbps || iters <= 0 || (q != 19 && q != 27) || (sbytes != 2 && sbytes != 4 && sbytes != 8) ||
+        (vec != 1 && vec != 2 && vec != 4) || pattern < 0 || pattern > 2 || x <= 0 || y <= 0 || z <= 0 ||
+        x % vec != 0)
+        return fail(PLB_ERR_INVALID_ARGUMENT, "bad pattern benchmark arguments");
+    const uint64_t ncell = (uint64_t)x * y * z;
+    if (ncell >= (1ull << 32) / 4) return fail(PLB_ERR_INVALID_ARGUMENT, "pattern benchmark domain too large");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(PLB_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= n) return fail(PLB_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    PLB_HIP_TRY(hipSetDevice(device));
+    // the fused layout's population stride (plb_state_create's default pad)
+    const uint64_t ps = ((ncell + 63) / 64) * 64 + 2048 + 64;
+    const size_t pop_bytes = (size_t)q * ps * sbytes;
+    void *A = nullptr, *B = nullptr, *flags = nullptr, *mask = nullptr;
+    hipError_t e = hipMalloc(&A, pop_bytes);
+    if (e == hipSuccess) e = hipMalloc(&B, pop_bytes);
+    if (e == hipSuccess) e = hipMalloc(&flags, ncell);
+    if (e == hipSuccess) e = hipMalloc(&mask, ncell * 4);
+    hipStream_t st = nullptr;
+    if (e == hipSuccess) e = hipStreamCreate(&st);
+    hipEvent_t a = nullptr, b = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipMemsetAsync(A, 1, pop_bytes, st);
+    if (e == hipSuccess) e = hipMemsetAsync(B, 0, pop_bytes, st);
+    if (e == hipSuccess) e = hipMemsetAsync(flags, 0, ncell, st);
+    if (e == hipSuccess) e = hipMemsetAsync(mask, 0, ncell * 4, st);
+    float t = 0.f;
+    bool launched = true;
+    if (e == hipSuccess) {
+        for (int w = 0; w < 2 && launched; ++w)
+            launched = pattern_launch(q, sbytes, vec, pattern, A, B, (const uint8_t*)flags, (const uint32_t*)mask, x,
+                                      y, z, ps, ncell, st);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(a, st);
+        for (int i = 0; i < iters && e == hipSuccess && launched; ++i) {
+            pattern_launch(q, sbytes, vec, pattern, A, B, (const uint8_t*)flags, (const uint32_t*)mask, x, y, z, ps,
+                           ncell, st);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(b, st);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (st) (void)hipStreamDestroy(st);
+    for (void* p : {A, B, flags, mask})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? PLB_ERR_OOM : PLB_ERR_HIP, hipGetErrorString(e));
+    const double per = (double)t / iters;
+    *gbps = (double)ncell * (2.0 * q * sbytes + 1.0) / (per * 1e-3) / 1e9;
+    if (ms) *ms = per;
     return PLB_OK;
 }
 

@@ -124,7 +124,7 @@ int lit_flush(plb_state* s) {
                 rc = timed(s, PLB_STAGE_RESET_DENSITY, [&] { return lit_reset_density(s, &op.bc); });
                 break;
             default:
-                rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, op.tau, op.force, op.avx2 != 0); });
+                rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, op.tau, op.force); });
                 break;
         }
         if (rc) return rc;
@@ -140,12 +140,10 @@ bool lit_queue_is(const plb_state* s, std::initializer_list<LitOpKind> kinds) {
     return true;
 }
 
-LitOp lit_op(LitOpKind kind, const plb_boundary* bc = nullptr, double tau = 0.0, const double* force = nullptr,
-             int avx2 = 0) {
+LitOp lit_op(LitOpKind kind, const plb_boundary* bc = nullptr, double tau = 0.0, const double* force = nullptr) {
     LitOp op;
     std::memset(&op, 0, sizeof(op));
     op.kind = kind;
-    op.avx2 = avx2;
     if (bc) op.bc = *bc;
     op.tau = tau;
     if (force)
@@ -266,6 +264,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->force[0] = s->force[1] = s->force[2] = 0.0;
     s->configured = false;
     s->collide_pending = false;
+    s->lit_ntmp_folded = false;
     s->moments_valid = false;
     s->scratch = nullptr;
     s->scratch_bytes = 0;
@@ -288,7 +287,6 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->tune_xcd = -1;
     s->tune_index64 = 0;
     s->tune_lit_fuse = 1;
-    s->tune_lit_avx2 = 0;
     s->post_collision = false;
     for (double& v : s->stage_ms) v = 0.0;
 
@@ -451,8 +449,8 @@ int plb_collide_srt(plb_state* s, double tau, const double force[3]) {
             if ((rc = lit_flush(s))) return rc;
         if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
         if (!s->tune_lit_fuse)
-            return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, tau, force, s->tune_lit_avx2 != 0); });
-        s->lit_queue.push_back(lit_op(kLitCollide, nullptr, tau, force, s->tune_lit_avx2));
+            return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, tau, force); });
+        s->lit_queue.push_back(lit_op(kLitCollide, nullptr, tau, force));
         return PLB_OK;
     }
     if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
@@ -474,14 +472,13 @@ int plb_stream(plb_state* s, const char* stream_type) {
         s->lit_queue.clear();
         return timed(s, PLB_STAGE_COLLIDE,
                      [&] {
-                         return lit_moments_collide_push(s, q[3].tau, q[3].force, &q[1].bc, &q[2].bc,
-                                                         q[3].avx2 != 0);
+                         return lit_moments_collide_push(s, q[3].tau, q[3].force, &q[1].bc, &q[2].bc);
                      });
     }
     if (fold && lit_queue_is(s, {kLitCollide})) {
         const LitOp c = s->lit_queue[0];
         s->lit_queue.clear();
-        return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, c.tau, c.force, c.avx2 != 0); });
+        return timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, c.tau, c.force); });
     }
     // every other request finds the deferred calls done, as the reference's does
     rc = lit_flush(s);
@@ -601,16 +598,16 @@ int plb_step(plb_state* s, int nsteps) {
             // last are computed by the next step's collision kernel, which
             // stores rho and u as those stages would before it collides.
             const plb_boundary* bc = &s->bc;
-            const bool fold = s->tune_lit_fuse != 0, avx2 = s->tune_lit_avx2 != 0;
+            const bool fold = s->tune_lit_fuse != 0;
             if (fold && i > 0) {
                 if ((rc = timed(s, PLB_STAGE_COLLIDE,
-                                [&] { return lit_moments_collide_push(s, s->tau, F, bc, bc, avx2); })))
+                                [&] { return lit_moments_collide_push(s, s->tau, F, bc, bc); })))
                     return rc;
             } else if (fold) {
-                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, s->tau, F, avx2); })))
+                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide_push(s, s->tau, F); })))
                     return rc;
             } else {
-                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->tau, F, avx2); }))) return rc;
+                if ((rc = timed(s, PLB_STAGE_COLLIDE, [&] { return lit_collide(s, s->tau, F); }))) return rc;
                 if ((rc = timed(s, PLB_STAGE_STREAM, [&] { return lit_push(s); }))) return rc;
             }
             if ((rc = timed(s, PLB_STAGE_NOSLIP, [&] { return lit_noslip(s, bc); }))) return rc;

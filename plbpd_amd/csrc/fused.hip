@@ -426,28 +426,29 @@ __global__ void k_fpop(typename Prec<V>::S* __restrict__ f, H* __restrict__ host
                        uint32_t Y, uint32_t Z, uint64_t pstride, int reference_layout, int upload) {
     using P = Prec<V>;
     const uint64_t n = (uint64_t)X * Y * Z;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * Q) return;
-    const int k = (int)(t % Q);
-    const uint64_t c = t / Q;
-    uint64_t h = t;
-    if (reference_layout) {
-        const uint64_t x = c % X, y = (c / X) % Y, z = c / ((uint64_t)X * Y);
-        h = ((((z + 1) * (Y + 2)) + y + 1) * (X + 2) + x + 1) * Q + k;
-    }
-    if (upload) {
-        double v = (double)host[h];
-        if (P::shifted) v = v - Lat<Q>::w(k);
-        typename P::S s;
-        if (V == kF16S)
-            s = (typename P::S)((float)v * 0x1.0p15f);
-        else
-            s = (typename P::S)v;
-        f[(uint64_t)k * pstride + c] = s;
-    } else {
-        double v = (double)P::load(f[(uint64_t)k * pstride + c]);
-        if (P::shifted) v = v + Lat<Q>::w(k);
-        host[h] = (H)v;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * Q;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(t % Q);
+        const uint64_t c = t / Q;
+        uint64_t h = t;
+        if (reference_layout) {
+            const uint64_t x = c % X, y = (c / X) % Y, z = c / ((uint64_t)X * Y);
+            h = ((((z + 1) * (Y + 2)) + y + 1) * (X + 2) + x + 1) * Q + k;
+        }
+        if (upload) {
+            double v = (double)host[h];
+            if (P::shifted) v = v - Lat<Q>::w(k);
+            typename P::S s;
+            if (V == kF16S)
+                s = (typename P::S)((float)v * 0x1.0p15f);
+            else
+                s = (typename P::S)v;
+            f[(uint64_t)k * pstride + c] = s;
+        } else {
+            double v = (double)P::load(f[(uint64_t)k * pstride + c]);
+            if (P::shifted) v = v + Lat<Q>::w(k);
+            host[h] = (H)v;
+        }
     }
 }
 
@@ -456,25 +457,27 @@ template <class H>
 __global__ void k_fmom_out(const double* __restrict__ src, H* __restrict__ host, uint32_t X,
                            uint32_t Y, uint32_t Z, int vlen, int reference_layout) {
     const uint64_t n = (uint64_t)X * Y * Z;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * vlen) return;
-    const int v = (int)(t % vlen);
-    const uint64_t c = t / vlen;
-    uint64_t h = t;
-    if (reference_layout) {
-        const uint64_t x = c % X, y = (c / X) % Y, z = c / ((uint64_t)X * Y);
-        h = ((((z + 1) * (Y + 2)) + y + 1) * (X + 2) + x + 1) * vlen + v;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * vlen;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int v = (int)(t % vlen);
+        const uint64_t c = t / vlen;
+        uint64_t h = t;
+        if (reference_layout) {
+            const uint64_t x = c % X, y = (c / X) % Y, z = c / ((uint64_t)X * Y);
+            h = ((((z + 1) * (Y + 2)) + y + 1) * (X + 2) + x + 1) * vlen + v;
+        }
+        host[h] = (H)src[v * n + c];
     }
-    host[h] = (H)src[v * n + c];
 }
 
 template <class S>
 __global__ void k_fraw(const S* __restrict__ f, S* __restrict__ out, uint64_t n, uint64_t pstride, int q) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * q) return;
-    const int k = (int)(t % q);
-    const uint64_t c = t / q;
-    out[t] = f[(uint64_t)k * pstride + c];
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * q;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(t % q);
+        const uint64_t c = t / q;
+        out[t] = f[(uint64_t)k * pstride + c];
+    }
 }
 
 // Neighbour-solid mask for domains with interior solids: bit 0 = the cell
@@ -935,7 +938,7 @@ int fused_transfer(plb_state* s, int field, int layout, int dtype, void* host, b
                 constexpr int Q = decltype(qc)::value;
                 constexpr int V = decltype(vc)::value;
                 using S = typename Prec<V>::S;
-                const unsigned nb = blocks_for(n * Q, kBlock);
+                const unsigned nb = stride_blocks(n * Q, kBlock);
                 if (dtype == PLB_DTYPE_F64)
                     hipLaunchKernelGGL((k_fpop<Q, V, double>), dim3(nb), dim3(kBlock), 0, s->stream,
                                        (S*)s->buf[s->cur], (double*)stage, (uint32_t)s->X, (uint32_t)s->Y,
@@ -949,7 +952,7 @@ int fused_transfer(plb_state* s, int field, int layout, int dtype, void* host, b
             });
         } else {
             const double* src = field == PLB_FIELD_RHO ? (const double*)s->rho : (const double*)s->u;
-            const unsigned nb = blocks_for(n * vlen, kBlock);
+            const unsigned nb = stride_blocks(n * vlen, kBlock);
             if (dtype == PLB_DTYPE_F64)
                 hipLaunchKernelGGL((k_fmom_out<double>), dim3(nb), dim3(kBlock), 0, s->stream, src,
                                    (double*)stage, (uint32_t)s->X, (uint32_t)s->Y, (uint32_t)s->Z, vlen, ref);
@@ -983,7 +986,7 @@ int fused_download_raw(plb_state* s, void* host, size_t bytes) {
     if (bytes < need) return fail(PLB_ERR_INVALID_ARGUMENT, "raw buffer too small");
     void* stage = nullptr;
     PLB_HIP_TRY(hipMalloc(&stage, need));
-    const unsigned nb = blocks_for(s->ncell * s->q, kBlock);
+    const unsigned nb = stride_blocks(s->ncell * s->q, kBlock);
     if (s->sbytes == 8)
         hipLaunchKernelGGL((k_fraw<uint64_t>), dim3(nb), dim3(kBlock), 0, s->stream,
                            (const uint64_t*)s->buf[s->cur], (uint64_t*)stage, (uint64_t)s->ncell,

@@ -167,7 +167,10 @@ __global__ __launch_bounds__(64 * row_block_waves(WPR)) void k_pull_row(const ty
         }
     }
     const bool active = row_ok && __any(any_fluid);
-    if (active && (lane == 0 || lane == 63)) {
+    // A wavefront whose own cells are all solid still publishes its edge
+    // values: a diagonal population it loaded comes from another row, where
+    // the source cell can be fluid, and the neighbouring segments pull it.
+    if (row_ok && (lane == 0 || lane == 63)) {
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
             if (Lat<Q>::cx(k) < 0 && lane == 0) edge_first[wave][xslot<Q>(k)] = f[k][0];

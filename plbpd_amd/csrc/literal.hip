@@ -83,24 +83,14 @@ __global__ void k_init(LitDims g, R* n, R* ntmp, R* rho, R* u, double rho0, doub
 }
 
 // CollisionSRT::_collision_kernel (CollisionSRT.cpp:153-176) for population
-// k of one cell, with _get_cu (84-91): the post-collision value of old.
-// With AVX (FP32 only), the first 8 * floor(Q / 8) directions follow the
-// reference's AVX2 kernel instead (CollisionSRT.cpp:118-142, built by default,
-// src/CMakeLists.txt:48-51): single-precision fused multiply-adds in its order.
-template <int Q, class R, bool AVX = false>
+// k of one cell, with _get_cu (84-91): the post-collision value of old.  In
+// FP32 the reference's promotions are kept: w_k * rho in float, the bracket in
+// double (double literals), the blend's first product in float, its sum in
+// double, the result rounded to float.
+template <int Q, class R>
 __device__ inline R collide_value(int k, R old, R rl, const R (&uu)[3], R usq, R om, R omm) {
     const R cu = (k == 0) ? (R)0
                           : (R)Lat<Q>::cx(k) * uu[0] + (R)Lat<Q>::cy(k) * uu[1] + (R)Lat<Q>::cz(k) * uu[2];
-    if (std::is_same<R, float>::value && AVX && k < 8 * (Q / 8)) {
-        const float kusq = -1.5f * (float)usq;
-        const float cusq = (float)cu * (float)cu;
-        float neq = __builtin_fmaf(3.0f, (float)cu, 1.0f);
-        neq = __builtin_fmaf(4.5f, cusq, neq);
-        neq = neq + kusq;
-        neq = neq * (float)rl;
-        neq = neq * (float)Lat<Q>::w(k);
-        return (R)__builtin_fmaf((float)omm, (float)old, (float)om * neq);
-    }
     if (std::is_same<R, float>::value) {
         const float wr = (float)Lat<Q>::w(k) * (float)rl;
         const double neq =
@@ -125,7 +115,7 @@ __device__ inline void updated_u(R u0, R u1, R u2, R rl, R tau, R F0, R F1, R F2
 }
 
 // CollisionSRT::operator() (CollisionSRT.cpp:25-176): in place, every interior cell.
-template <int Q, class R, bool AVX>
+template <int Q, class R>
 __global__ void k_collide(LitDims g, R* __restrict__ n, const R* __restrict__ rho,
                           const R* __restrict__ u, R tau, R om, R omm, R F0, R F1, R F2) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -138,17 +128,51 @@ __global__ void k_collide(LitDims g, R* __restrict__ n, const R* __restrict__ rh
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         R* p = n + k * g.ncell + c;
-        *p = collide_value<Q, R, AVX>(k, *p, rl, uu, usq, om, omm);
+        *p = collide_value<Q, R>(k, *p, rl, uu, usq, om, omm);
+    }
+}
+
+// Whether interior cell (x, y, z) lies on the outer interior layer (a face
+// layer of Boundary.cpp:113-148 on any axis).  The collide+push kernels store
+// their in-place post-collision values only there.  Off that layer, the
+// value the reference leaves in place (it becomes ntmp after the swap) is
+// the one the same kernel pushed to c + c_k, and nothing reads it before the
+// next push overwrites it: the next push writes ntmp(c', k) for every c' with
+// c' - c_k interior, which is every cell off the outer layer.  Between the
+// two pushes only a read of ntmp or a change of n at those pushed slots
+// (a collision in place, an upload) needs the values, and k_ntmp_restore
+// copies them back from n first.  No-slip and periodic copies never change
+// a slot n(c + c_k, k) of such a cell: no-slip writes n(f + c_kp, kp) for face
+// cells f, periodic copies write n(c', k) with c' - c_k in the halo.
+__device__ inline bool outer_layer(const LitDims& g, int x, int y, int z) {
+    return x == 1 || x == g.X || y == 1 || y == g.Y || z == 1 || z == g.Z;
+}
+
+// ntmp(c, k) = n(c + c_k, k) for the interior cells off the outer layer: the
+// in-place values the collide+push kernels did not store (see outer_layer).
+template <int Q, class R>
+__global__ void k_ntmp_restore(LitDims g, const R* __restrict__ n, R* __restrict__ t) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int y = blockIdx.y + 1, z = blockIdx.z + 1;
+    if (x > g.X || outer_layer(g, x, y, z)) return;
+    const uint64_t c = g.cell(z, y, x);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int64_t off = (int64_t)Lat<Q>::cx(k) + (int64_t)Lat<Q>::cy(k) * g.nx +
+                            (int64_t)Lat<Q>::cz(k) * g.nx * g.ny;
+        t[k * g.ncell + c] = n[k * g.ncell + (uint64_t)((int64_t)c + off)];
     }
 }
 
 // The collision followed by Streaming::_push_kernel (Streaming.cpp:63-74) in
-// one pass: each post-collision value is stored in place, as k_collide leaves
-// it (it becomes ntmp after the swap, and its entries that the next push
-// does not overwrite are read again), and at x + c_k in t, as k_push copies
-// it.  Both buffers end exactly as after k_collide then k_push; the push's
-// re-read of n is saved.
-template <int Q, class R, bool AVX>
+// one pass: each post-collision value is stored at x + c_k in t, as k_push
+// copies it, and in place on the outer layer, as k_collide leaves it (it
+// becomes ntmp after the swap; the next push leaves those entries, and the
+// reference's stage order reads some of them back, Q1).  Every value a caller
+// or a later stage can read equals the one k_collide then k_push give (see
+// outer_layer); the push's re-read of n and the in-place stores off the outer
+// layer are saved.
+template <int Q, class R>
 __global__ void k_collide_push(LitDims g, R* __restrict__ n, const R* __restrict__ rho,
                                const R* __restrict__ u, R* __restrict__ t, R tau, R om, R omm, R F0, R F1,
                                R F2) {
@@ -157,12 +181,13 @@ __global__ void k_collide_push(LitDims g, R* __restrict__ n, const R* __restrict
     if (x > g.X) return;
     const uint64_t c = g.cell(z, y, x);
     const R rl = rho[c];
+    const bool keep = outer_layer(g, x, y, z);
     R uu[3], usq;
     updated_u(u[c], u[g.ncell + c], u[2 * g.ncell + c], rl, tau, F0, F1, F2, uu, usq);
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        const R v = collide_value<Q, R, AVX>(k, n[k * g.ncell + c], rl, uu, usq, om, omm);
-        n[k * g.ncell + c] = v;
+        const R v = collide_value<Q, R>(k, n[k * g.ncell + c], rl, uu, usq, om, omm);
+        if (keep) n[k * g.ncell + c] = v;
         const int64_t off = (int64_t)Lat<Q>::cx(k) + (int64_t)Lat<Q>::cy(k) * g.nx +
                             (int64_t)Lat<Q>::cz(k) * g.nx * g.ny;
         t[k * g.ncell + (uint64_t)((int64_t)c + off)] = v;
@@ -391,8 +416,8 @@ __device__ inline void apply_resets(const FaceResets<R>& fr, const LitDims& g, i
 // calls that the C ABI deferred until the push.
 // rho and u are computed from n exactly as k_moments and the resets leave
 // them, stored (no-slip reads rho before the next moments), and used by the
-// collision at once; n and t end as after k_collide_push.
-template <int Q, class R, bool AVX>
+// collision at once; n and t end as k_collide_push leaves them.
+template <int Q, class R>
 __global__ void k_moments_collide_push(LitDims g, R* __restrict__ n, R* __restrict__ rho, R* __restrict__ u,
                                        R* __restrict__ t, FaceResets<R> fr, R tau, R om, R omm, R F0, R F1,
                                        R F2) {
@@ -414,12 +439,13 @@ __global__ void k_moments_collide_push(LitDims g, R* __restrict__ n, R* __restri
     u[c] = um[0];
     u[g.ncell + c] = um[1];
     u[2 * g.ncell + c] = um[2];
+    const bool keep = outer_layer(g, x, y, z);
     R uu[3], usq;
     updated_u(um[0], um[1], um[2], rl, tau, F0, F1, F2, uu, usq);
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        const R v = collide_value<Q, R, AVX>(k, nv[k], rl, uu, usq, om, omm);
-        n[k * g.ncell + c] = v;
+        const R v = collide_value<Q, R>(k, nv[k], rl, uu, usq, om, omm);
+        if (keep) n[k * g.ncell + c] = v;
         const int64_t off = (int64_t)Lat<Q>::cx(k) + (int64_t)Lat<Q>::cy(k) * g.nx +
                             (int64_t)Lat<Q>::cz(k) * g.nx * g.ny;
         t[k * g.ncell + (uint64_t)((int64_t)c + off)] = v;
@@ -431,19 +457,20 @@ __global__ void k_moments_collide_push(LitDims g, R* __restrict__ n, R* __restri
 template <class H, class R>
 __global__ void k_transfer(LitDims g, H* __restrict__ host, R* __restrict__ dev, int vlen,
                            int compact, int upload, uint64_t total) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int v = (int)(t % vlen);
-    const uint64_t hc = t / vlen;
-    uint64_t c = hc;
-    if (compact) {
-        const int x = (int)(hc % g.X), y = (int)((hc / g.X) % g.Y), z = (int)(hc / ((uint64_t)g.X * g.Y));
-        c = g.cell(z + 1, y + 1, x + 1);
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int v = (int)(t % vlen);
+        const uint64_t hc = t / vlen;
+        uint64_t c = hc;
+        if (compact) {
+            const int x = (int)(hc % g.X), y = (int)((hc / g.X) % g.Y), z = (int)(hc / ((uint64_t)g.X * g.Y));
+            c = g.cell(z + 1, y + 1, x + 1);
+        }
+        if (upload)
+            dev[v * g.ncell + c] = (R)host[t];
+        else
+            host[t] = (H)dev[v * g.ncell + c];
     }
-    if (upload)
-        dev[v * g.ncell + c] = (R)host[t];
-    else
-        host[t] = (H)dev[v * g.ncell + c];
 }
 
 // FP64 sums over interior cells of rho and rho*u (two deterministic passes).
@@ -503,6 +530,23 @@ dim3 interior_grid(const plb_state* s) {
     return dim3((unsigned)((s->X + b - 1) / b), (unsigned)s->Y, (unsigned)s->Z);
 }
 
+// Stores the in-place values the last collide+push kernel left out (see
+// outer_layer), before anything reads ntmp or changes n.
+int lit_ntmp_restore(plb_state* s) {
+    if (!s->lit_ntmp_folded) return PLB_OK;
+    const LitDims g = dims_of(s);
+    const int rc = with_qr(s, [&](auto qc, auto rr) {
+        constexpr int Q = decltype(qc)::value;
+        using R = decltype(rr);
+        hipLaunchKernelGGL((k_ntmp_restore<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
+                           (const R*)s->buf[s->cur], (R*)s->buf[s->cur ^ 1]);
+        PLB_HIP_TRY(hipGetLastError());
+        return PLB_OK;
+    });
+    if (rc == PLB_OK) s->lit_ntmp_folded = false;
+    return rc;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ launchers ----
@@ -519,6 +563,7 @@ int lit_initialize(plb_state* s, double rho0, const double u0[3], const plb_init
                            s->stream, g, (R*)s->buf[s->cur], (R*)s->buf[s->cur ^ 1], (R*)s->rho,
                            (R*)s->u, rho0, u0[0], u0[1], u0[2], kind, seed, amp, tg_dev);
         PLB_HIP_TRY(hipGetLastError());
+        s->lit_ntmp_folded = false;
         return PLB_OK;
     });
 }
@@ -536,50 +581,45 @@ void relaxation(double tau, R* tr, R* om, R* omm) {
 }
 }  // namespace
 
-int lit_collide(plb_state* s, double tau, const double F[3], bool avx2) {
+int lit_collide(plb_state* s, double tau, const double F[3]) {
     const LitDims g = dims_of(s);
+    const int rc = lit_ntmp_restore(s);  // the collision changes the n that ntmp is kept in
+    if (rc) return rc;
     return with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
         using R = decltype(rr);
         R tr, om, omm;
         relaxation(tau, &tr, &om, &omm);
-        if (avx2 && std::is_same<R, float>::value)
-            hipLaunchKernelGGL((k_collide<Q, R, true>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
-                               (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, tr, om, omm, (R)F[0], (R)F[1],
-                               (R)F[2]);
-        else
-            hipLaunchKernelGGL((k_collide<Q, R, false>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
-                               (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, tr, om, omm, (R)F[0], (R)F[1],
-                               (R)F[2]);
+        hipLaunchKernelGGL((k_collide<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
+                           (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, tr, om, omm, (R)F[0], (R)F[1],
+                           (R)F[2]);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
 }
 
-int lit_collide_push(plb_state* s, double tau, const double F[3], bool avx2) {
+int lit_collide_push(plb_state* s, double tau, const double F[3]) {
     const LitDims g = dims_of(s);
     const int rc = with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
         using R = decltype(rr);
         R tr, om, omm;
         relaxation(tau, &tr, &om, &omm);
-        if (avx2 && std::is_same<R, float>::value)
-            hipLaunchKernelGGL((k_collide_push<Q, R, true>), interior_grid(s), dim3(interior_block(s)), 0, s->stream,
-                               g, (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, (R*)s->buf[s->cur ^ 1], tr,
-                               om, omm, (R)F[0], (R)F[1], (R)F[2]);
-        else
-            hipLaunchKernelGGL((k_collide_push<Q, R, false>), interior_grid(s), dim3(interior_block(s)), 0,
-                               s->stream, g, (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u,
-                               (R*)s->buf[s->cur ^ 1], tr, om, omm, (R)F[0], (R)F[1], (R)F[2]);
+        hipLaunchKernelGGL((k_collide_push<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream, g,
+                           (R*)s->buf[s->cur], (const R*)s->rho, (const R*)s->u, (R*)s->buf[s->cur ^ 1], tr, om,
+                           omm, (R)F[0], (R)F[1], (R)F[2]);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
-    if (rc == PLB_OK) s->cur ^= 1;  // std::swap(simdata.ntmp, simdata.n)
+    if (rc == PLB_OK) {
+        s->cur ^= 1;  // std::swap(simdata.ntmp, simdata.n)
+        s->lit_ntmp_folded = true;
+    }
     return rc;
 }
 
 int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc_vel,
-                             const plb_boundary* bc_rho, bool avx2) {
+                             const plb_boundary* bc_rho) {
     const LitDims g = dims_of(s);
     const int rc = with_qr(s, [&](auto qc, auto rr) {
         constexpr int Q = decltype(qc)::value;
@@ -593,18 +633,16 @@ int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const 
             for (int i = 0; i < 3; ++i) fr.vel[f][i] = (R)bc_vel->vel[f][i];
         }
         fr.rho_s = (R)bc_rho->rho_s;
-        if (avx2 && std::is_same<R, float>::value)
-            hipLaunchKernelGGL((k_moments_collide_push<Q, R, true>), interior_grid(s), dim3(interior_block(s)), 0,
-                               s->stream, g, (R*)s->buf[s->cur], (R*)s->rho, (R*)s->u, (R*)s->buf[s->cur ^ 1], fr,
-                               tr, om, omm, (R)F[0], (R)F[1], (R)F[2]);
-        else
-            hipLaunchKernelGGL((k_moments_collide_push<Q, R, false>), interior_grid(s), dim3(interior_block(s)), 0,
-                               s->stream, g, (R*)s->buf[s->cur], (R*)s->rho, (R*)s->u, (R*)s->buf[s->cur ^ 1], fr,
-                               tr, om, omm, (R)F[0], (R)F[1], (R)F[2]);
+        hipLaunchKernelGGL((k_moments_collide_push<Q, R>), interior_grid(s), dim3(interior_block(s)), 0, s->stream,
+                           g, (R*)s->buf[s->cur], (R*)s->rho, (R*)s->u, (R*)s->buf[s->cur ^ 1], fr, tr, om, omm,
+                           (R)F[0], (R)F[1], (R)F[2]);
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
-    if (rc == PLB_OK) s->cur ^= 1;  // std::swap(simdata.ntmp, simdata.n)
+    if (rc == PLB_OK) {
+        s->cur ^= 1;  // std::swap(simdata.ntmp, simdata.n)
+        s->lit_ntmp_folded = true;
+    }
     return rc;
 }
 
@@ -618,7 +656,10 @@ int lit_push(plb_state* s) {
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
-    if (rc == PLB_OK) s->cur ^= 1;  // std::swap(simdata.ntmp, simdata.n)
+    if (rc == PLB_OK) {
+        s->cur ^= 1;  // std::swap(simdata.ntmp, simdata.n)
+        s->lit_ntmp_folded = false;  // the push wrote every cell off the outer layer
+    }
     return rc;
 }
 
@@ -719,6 +760,10 @@ int lit_reset_density(plb_state* s, const plb_boundary* bc) {
 
 int lit_transfer(plb_state* s, int field, int layout, int dtype, void* host, bool upload) {
     const LitDims g = dims_of(s);
+    if ((field == PLB_FIELD_NTMP && !upload) || (field == PLB_FIELD_N && upload)) {
+        const int r = lit_ntmp_restore(s);
+        if (r) return r;
+    }
     int vlen;
     void* dev;
     switch (field) {
@@ -743,11 +788,11 @@ int lit_transfer(plb_state* s, int field, int layout, int dtype, void* host, boo
         rc = with_qr(s, [&](auto, auto rr) {
             using R = decltype(rr);
             if (dtype == PLB_DTYPE_F64)
-                hipLaunchKernelGGL((k_transfer<double, R>), dim3(blocks_for(total, kThreads)),
+                hipLaunchKernelGGL((k_transfer<double, R>), dim3(stride_blocks(total, kThreads)),
                                    dim3(kThreads), 0, s->stream, g, (double*)stage, (R*)dev, vlen,
                                    compact, upload ? 1 : 0, total);
             else
-                hipLaunchKernelGGL((k_transfer<float, R>), dim3(blocks_for(total, kThreads)),
+                hipLaunchKernelGGL((k_transfer<float, R>), dim3(stride_blocks(total, kThreads)),
                                    dim3(kThreads), 0, s->stream, g, (float*)stage, (R*)dev, vlen,
                                    compact, upload ? 1 : 0, total);
             PLB_HIP_TRY(hipGetLastError());
@@ -761,6 +806,7 @@ int lit_transfer(plb_state* s, int field, int layout, int dtype, void* host, boo
     hipError_t e = hipStreamSynchronize(s->stream);
     (void)hipFree(stage);
     if (rc == PLB_OK && e != hipSuccess) rc = fail(PLB_ERR_HIP, hipGetErrorString(e));
+    if (rc == PLB_OK && upload && field == PLB_FIELD_NTMP) s->lit_ntmp_folded = false;
     return rc;
 }
 

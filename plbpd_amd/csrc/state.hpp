@@ -53,7 +53,6 @@ struct LitOp {
     LitOpKind kind;
     double tau;        // kLitCollide
     double force[3];   // kLitCollide
-    int avx2;          // kLitCollide: PLB_TUNE_LITERAL_AVX2 at the call
     plb_boundary bc;   // the resets
 };
 
@@ -83,6 +82,9 @@ struct plb_state {
     // its kernel (at most moments, reset velocity, reset density, collide, in
     // that order); any other call runs them first, in order
     std::vector<plb::LitOp> lit_queue;
+    // literal: the last collide+push kernel stored its in-place values on the
+    // outer layer only; ntmp off that layer is n(c + c_k, k) until restored
+    bool lit_ntmp_folded;
     bool moments_valid;
     void* scratch;  // reductions, face staging
     size_t scratch_bytes;
@@ -100,7 +102,6 @@ struct plb_state {
     int tune_index64;     // 1: 64-bit index kernels even where 32-bit offsets fit
     int tune_vector;      // row pull kernel cells per lane (1, 2, 4) where X allows; 0 off; -1 auto
     int tune_lit_fuse;    // literal: 1 collision and push in one kernel when they follow each other
-    int tune_lit_avx2;    // literal FP32: 1 collision arithmetic of the reference's AVX2 kernel
     bool post_collision;  // fused: buf[cur] holds post-collision populations (pull step)
     // timers
     bool timers_on;
@@ -118,13 +119,12 @@ int timer_end(plb_state* s, TimerPair* tp);
 // literal mode (literal.hip)
 int lit_initialize(plb_state* s, double rho0, const double u0[3], const plb_init_spec* init,
                    const double* tg_dev);
-// avx2: FP32 collision in the arithmetic of the reference's AVX2 kernel
-int lit_collide(plb_state* s, double tau, const double F[3], bool avx2);
+int lit_collide(plb_state* s, double tau, const double F[3]);
 int lit_push(plb_state* s);
-int lit_collide_push(plb_state* s, double tau, const double F[3], bool avx2);
+int lit_collide_push(plb_state* s, double tau, const double F[3]);
 // bc_vel: the faces whose velocity is reset; bc_rho: those whose density is
 int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc_vel,
-                             const plb_boundary* bc_rho, bool avx2);
+                             const plb_boundary* bc_rho);
 int lit_noslip(plb_state* s, const plb_boundary* bc);
 int lit_periodic(plb_state* s, const plb_boundary* bc);
 int lit_moments(plb_state* s);
@@ -201,5 +201,13 @@ __host__ __device__ inline R feq_ref(int k, R rho, const R u[3], R usq) {
 }
 
 inline unsigned blocks_for(size_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// Grid of a grid-stride kernel over n items: one item per thread up to 2^20
+// blocks, more items per thread beyond (a launch holds fewer than 2^32
+// work-items, and a layout transfer of a large state has more values).
+inline unsigned stride_blocks(uint64_t n, unsigned threads) {
+    const uint64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b < (1u << 20) ? (b ? b : 1) : (1u << 20));
+}
 
 }  // namespace plb

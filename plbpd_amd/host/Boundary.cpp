@@ -1,27 +1,9 @@
 #include "Boundary.hpp"
 
-#include <chrono>
 #include <cstring>
 
-namespace chrono = std::chrono;
-
-float Boundary::_time_noslip = 0.0f;
-float Boundary::_time_periodicity = 0.0f;
-float Boundary::_time_reset = 0.0f;
-float Boundary::_time_velocity_reset = 0.0f;
-float Boundary::_time_density_reset = 0.0f;
 // face order of the reference (Boundary.cpp:17-18) = PLB_EAST ... PLB_DOWN
 const std::vector<std::string> Boundary::_directions = {"east", "west", "north", "south", "up", "down"};
-
-namespace {
-template <class Fn>
-float timed(Fn&& fn) {
-    const auto start = chrono::system_clock::now();
-    fn();
-    const chrono::duration<float> elapsed = chrono::system_clock::now() - start;
-    return elapsed.count();
-}
-}  // namespace
 
 Boundary::Boundary(const LBModel* lbmodel, const Domain* domain, const float solid_density, BoundaryType type,
                    BoundaryVelocity velocity)
@@ -50,35 +32,43 @@ const BoundaryType Boundary::get_boundary_type() const { return _type; }
 const BoundaryVelocity Boundary::get_boundary_velocity() const { return _velocity; }
 
 void Boundary::reset(SimData& simdata) const {
-    _time_reset += timed([&] {
-        reset_velocity(simdata);
-        reset_density(simdata);
-    });
+    reset_velocity(simdata);
+    reset_density(simdata);
 }
 
 void Boundary::reset_velocity(SimData& simdata) const {
-    _time_velocity_reset += timed([&] { plbpd::check(plb_reset_velocity(simdata.device(), &bc_)); });
+    plbpd::check(plb_reset_velocity(simdata.device(), &bc_));
 }
 
 void Boundary::reset_density(SimData& simdata) const {
-    _time_density_reset += timed([&] { plbpd::check(plb_reset_density(simdata.device(), &bc_)); });
+    plbpd::check(plb_reset_density(simdata.device(), &bc_));
 }
 
 void Boundary::apply_periodicity(SimData& simdata) const {
-    _time_periodicity += timed([&] { plbpd::check(plb_apply_periodicity(simdata.device(), &bc_)); });
+    plbpd::check(plb_apply_periodicity(simdata.device(), &bc_));
 }
 
-void Boundary::apply_noslip(SimData& simdata) const {
-    _time_noslip += timed([&] { plbpd::check(plb_apply_noslip(simdata.device(), &bc_)); });
-}
+void Boundary::apply_noslip(SimData& simdata) const { plbpd::check(plb_apply_noslip(simdata.device(), &bc_)); }
 
 void Boundary::set_solid_mask(SimData& simdata, const uint8_t* mask) const {
     plbpd::check(plb_set_boundary(simdata.device(), &bc_, mask));
 }
 
-float Boundary::get_time_noslip() const { return _time_noslip; }
-float Boundary::get_time_periodicity() const { return _time_periodicity; }
-float Boundary::get_time_reset() const { return _time_reset; }
-float Boundary::get_time_velocity_reset() const { return _time_velocity_reset; }
-float Boundary::get_time_density_reset() const { return _time_density_reset; }
-float Boundary::get_total_time() const { return _time_noslip + _time_periodicity + _time_reset; }
+// The timers report GPU seconds of the stages on every device state
+// (resets folded into a literal collide + push kernel count under the
+// collision).  reset covers both resets, as Boundary.cpp:58-66 times them.
+float Boundary::get_time_noslip() const { return (float)plbpd::stage_seconds()[PLB_STAGE_NOSLIP]; }
+float Boundary::get_time_periodicity() const { return (float)plbpd::stage_seconds()[PLB_STAGE_PERIODIC]; }
+float Boundary::get_time_velocity_reset() const {
+    return (float)plbpd::stage_seconds()[PLB_STAGE_RESET_VELOCITY];
+}
+float Boundary::get_time_density_reset() const { return (float)plbpd::stage_seconds()[PLB_STAGE_RESET_DENSITY]; }
+float Boundary::get_time_reset() const {
+    const auto t = plbpd::stage_seconds();
+    return (float)(t[PLB_STAGE_RESET_VELOCITY] + t[PLB_STAGE_RESET_DENSITY]);
+}
+float Boundary::get_total_time() const {
+    const auto t = plbpd::stage_seconds();
+    return (float)(t[PLB_STAGE_NOSLIP] + t[PLB_STAGE_PERIODIC] + t[PLB_STAGE_RESET_VELOCITY] +
+                   t[PLB_STAGE_RESET_DENSITY]);
+}

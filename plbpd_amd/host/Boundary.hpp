@@ -2,6 +2,7 @@
 // face maps are turned into a plb_boundary once; every call runs on the
 // device.  A no-slip face without a velocity entry throws std::out_of_range
 // from apply_noslip, as the reference's _velocity.at does (Boundary.cpp:197).
+// The timer getters report GPU time (HIP events on every device state).
 #ifndef PLBPD_AMD_HOST_BOUNDARY_HPP_
 #define PLBPD_AMD_HOST_BOUNDARY_HPP_
 
@@ -53,11 +54,6 @@ class Boundary final {
     BoundaryType _type;
     BoundaryVelocity _velocity;
     plb_boundary bc_;
-    static float _time_noslip;
-    static float _time_periodicity;
-    static float _time_reset;
-    static float _time_velocity_reset;
-    static float _time_density_reset;
     static const std::vector<std::string> _directions;
 };
 

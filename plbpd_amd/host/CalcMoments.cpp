@@ -1,19 +1,13 @@
 #include "CalcMoments.hpp"
 
-namespace chrono = std::chrono;
-
-float CalcMoments::_time_calc_moment = 0.0f;
-
 CalcMoments::CalcMoments(const LBModel* lbmodel) { (void)lbmodel; }
 CalcMoments::~CalcMoments() {}
-float CalcMoments::get_total_time() const { return _time_calc_moment; }
 
-void CalcMoments::operator()(SimData& simdata) const {
-    const auto start = chrono::system_clock::now();
-    plbpd::check(plb_calc_moments(simdata.device()));
-    const chrono::duration<float> elapsed = chrono::system_clock::now() - start;
-    _time_calc_moment += elapsed.count();
-}
+// GPU seconds of the moment passes run alone (a literal pass folded into the
+// next collide + push kernel counts under the collision).
+float CalcMoments::get_total_time() const { return (float)plbpd::stage_seconds()[PLB_STAGE_MOMENTS]; }
+
+void CalcMoments::operator()(SimData& simdata) const { plbpd::check(plb_calc_moments(simdata.device())); }
 
 std::array<double, 4> CalcMoments::global_moments(SimData& simdata) const {
     std::array<double, 4> out{};

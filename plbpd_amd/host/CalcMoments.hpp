@@ -5,7 +5,6 @@
 #define PLBPD_AMD_HOST_CALCMOMENTS_HPP_
 
 #include <array>
-#include <chrono>
 
 #include "LBModel.hpp"
 #include "SimData.hpp"
@@ -20,9 +19,6 @@ class CalcMoments final {
     void operator()(SimData& simdata) const;
     // FP64 sums over fluid cells: {sum rho, sum rho ux, sum rho uy, sum rho uz}.
     std::array<double, 4> global_moments(SimData& simdata) const;
-
- private:
-    static float _time_calc_moment;
 };
 
 #endif  // PLBPD_AMD_HOST_CALCMOMENTS_HPP_

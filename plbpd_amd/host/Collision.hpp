@@ -1,4 +1,6 @@
 // Drop-in Collision base (interface of the reference's Collision.hpp:10-21).
+// The static timer counts GPU time: get_total_time adds the HIP-event time of
+// every collision on a device state to what subclasses add to _time.
 #ifndef PLBPD_AMD_HOST_COLLISION_HPP_
 #define PLBPD_AMD_HOST_COLLISION_HPP_
 

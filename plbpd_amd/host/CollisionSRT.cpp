@@ -1,7 +1,5 @@
 #include "CollisionSRT.hpp"
 
-namespace chrono = std::chrono;
-
 CollisionSRT::CollisionSRT(const LBModel* lbmodel, const Domain* domain)
     : Collision(), tau_(domain->get_relaxation_time()), force_(domain->get_external_force_f64()) {
     (void)lbmodel;
@@ -9,14 +7,8 @@ CollisionSRT::CollisionSRT(const LBModel* lbmodel, const Domain* domain)
 
 CollisionSRT::~CollisionSRT() {}
 
+// Its GPU time is counted by the state's timers (Collision::get_total_time).
 void CollisionSRT::operator()(SimData& simdata, bool reference) const {
-    const auto start = chrono::system_clock::now();
-    // The reference's default loop (reference = false) is its AVX2 kernel in
-    // the default build; FP32 literal states reproduce its rounding.
-    const plbpd::DeviceConfig& cfg = simdata.device_config();
-    if (cfg.mode == PLB_MODE_LITERAL && cfg.variant == PLB_F32)
-        plbpd::check(plb_set_tuning(simdata.device(), PLB_TUNE_LITERAL_AVX2, reference ? 0 : 1));
+    (void)reference;  // the serial and parallel loops compute the same values
     plbpd::check(plb_collide_srt(simdata.device(), tau_, force_.data()));
-    const chrono::duration<float> elapsed = chrono::system_clock::now() - start;
-    Collision::_time += elapsed.count();
 }

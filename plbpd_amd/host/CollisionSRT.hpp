@@ -1,12 +1,11 @@
 // Drop-in CollisionSRT (interface of the reference's CollisionSRT.hpp:11-45):
 // BGK with the velocity-shift body force, on the device
 // (plb_collide_srt).  `reference` selects the serial or the TBB loop in the
-// reference (CollisionSRT.cpp:25-33).  In FP64 the two agree and run the
-// same GPU kernel.  In an FP32 literal state `reference` = true gives the
-// serial kernel's arithmetic (CollisionSRT.cpp:153-176) and false, the
-// default, that of the AVX2 kernel of the reference's default build (FP32
-// fused multiply-adds for the first 8 * floor(q / 8) directions,
-// CollisionSRT.cpp:93-149), through PLB_TUNE_LITERAL_AVX2.
+// reference (CollisionSRT.cpp:25-33); both loops call the same scalar kernel
+// in a build without AVX2 (CollisionSRT.cpp:60-64, 153-176), so both values
+// run the same GPU kernel, in that kernel's arithmetic (in FP32, its
+// float/double promotions).  The AVX2 kernel of builds with -DAVX2 -mfma
+// rounds differently and is not reproduced.
 #ifndef PLBPD_AMD_HOST_COLLISIONSRT_HPP_
 #define PLBPD_AMD_HOST_COLLISIONSRT_HPP_
 

@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <iostream>
+#include <string>
 
 PLBPD::PLBPD(const LBModel* lbmodel, const Domain* domain, const Boundary* boundary)
     : _domain(domain), _boundary(boundary), _lbdynamics(new LBDynamics(lbmodel, domain)),
@@ -29,17 +30,26 @@ void PLBPD::march_in_time(size_t num_timesteps, SimData& simdata) const {
     std::cout << "Time taken by timeloop: " << elapsed.count() << "s" << std::endl;
 }
 
+// The class timers, one line each, nested by how the reference groups them
+// (a leading '-' per level).  Their values are GPU seconds (HIP events).
 void PLBPD::print_times() const {
-    std::cout << "LBDynamics: " << _lbdynamics->get_total_time() << "s\n";
-    std::cout << "-collide: " << _lbdynamics->get_time_collide() << "s\n";
-    std::cout << "-stream: " << _lbdynamics->get_time_stream() << "s\n";
-    std::cout << "CalcMoments: " << _calc_moments->get_total_time() << "s\n";
-    std::cout << "Boundary: " << _boundary->get_total_time() << "s\n";
-    std::cout << "-noslip: " << _boundary->get_time_noslip() << "s\n";
-    std::cout << "-periodicity: " << _boundary->get_time_periodicity() << "s\n";
-    std::cout << "-reset: " << _boundary->get_time_reset() << "s\n";
-    std::cout << "--velocity-reset: " << _boundary->get_time_velocity_reset() << "s\n";
-    std::cout << "--density-reset: " << _boundary->get_time_density_reset() << "s\n";
+    struct Row {
+        int depth;
+        const char* name;
+        float seconds;
+    };
+    const Boundary& b = *_boundary;
+    const Row rows[] = {{0, "LBDynamics", _lbdynamics->get_total_time()},
+                        {1, "collide", _lbdynamics->get_time_collide()},
+                        {1, "stream", _lbdynamics->get_time_stream()},
+                        {0, "CalcMoments", _calc_moments->get_total_time()},
+                        {0, "Boundary", b.get_total_time()},
+                        {1, "noslip", b.get_time_noslip()},
+                        {1, "periodicity", b.get_time_periodicity()},
+                        {1, "reset", b.get_time_reset()},
+                        {2, "velocity-reset", b.get_time_velocity_reset()},
+                        {2, "density-reset", b.get_time_density_reset()}};
+    for (const Row& r : rows) std::cout << std::string(r.depth, '-') << r.name << ": " << r.seconds << "s\n";
 }
 
 void PLBPD::print_device_times(SimData& simdata) const {

@@ -21,11 +21,12 @@ class PLBPD final {
     ~PLBPD();
     void initialize(SimData& simdata) const;
     void march_in_time(size_t num_timesteps, SimData& simdata) const;
+    // The reference's per-class timers (PLBPD.cpp:37-48), in GPU seconds
+    // summed over every device state of the process.
     void print_times() const;
-    // GPU time per stage on this state (HIP events), in seconds, accumulated
-    // while its timers are on (plbpd::DeviceConfig::device_timers); the
-    // reference-interface timers of print_times measure the host calls, which
-    // return before the GPU work ends.
+    // GPU time per stage of this state alone (HIP events), in seconds,
+    // accumulated while its timers are on (plbpd::DeviceConfig::device_timers,
+    // the default).  No reference counterpart.
     void print_device_times(SimData& simdata) const;
 
  private:

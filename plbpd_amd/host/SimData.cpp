@@ -3,6 +3,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <type_traits>
+#include <vector>
 
 #include "plbpd_state.hpp"
 
@@ -35,7 +36,10 @@ SimData::~SimData() {
     delete u;
     delete n;
     delete ntmp;
-    if (state_) plb_state_destroy(state_);
+    if (state_) {
+        plbpd::retire_state(state_);
+        plb_state_destroy(state_);
+    }
 }
 
 void SimData::bootstrap() {
@@ -61,6 +65,7 @@ void SimData::bootstrap() {
         d.pop_pad = cfg_.pop_pad;
         plbpd::check(plb_state_create(&d, &state_));
         if (cfg_.device_timers) plbpd::check(plb_enable_timers(state_, 1));
+        plbpd::register_state(state_);
     } catch (...) {
         delete rho;
         delete u;
@@ -70,6 +75,8 @@ void SimData::bootstrap() {
         u = nullptr;
         n = nullptr;
         ntmp = nullptr;
+        if (state_) plb_state_destroy(state_);
+        state_ = nullptr;
         throw;
     }
 }
@@ -91,17 +98,36 @@ void SimData::sync_to_device() {
     }
 }
 
+// The file's datasets come from FP64 downloads of the device state, so an
+// FP64 state is written exactly whatever the host mirror's type; the mirror
+// is refreshed as well (the reference writes the fields it holds).
 void SimData::write_state(std::string dump_file) {
     sync_to_host();
     size_t X, Y, Z;
     std::tie(X, Y, Z) = dims_;
-    plbpd::write_state_file(dump_file, X, Y, Z, kdim_, rho->data(), u->data(), n->data(),
-                            cfg_.mode == PLB_MODE_LITERAL ? ntmp->data() : nullptr);
+    const size_t cells = (X + 2) * (Y + 2) * (Z + 2);
+    const bool literal = cfg_.mode == PLB_MODE_LITERAL;
+    std::vector<double> r(cells), v(3 * cells), f(kdim_ * cells), t(literal ? kdim_ * cells : 0);
+    plbpd::check(plb_download(state_, PLB_FIELD_RHO, PLB_LAYOUT_REFERENCE, PLB_DTYPE_F64, r.data()));
+    plbpd::check(plb_download(state_, PLB_FIELD_U, PLB_LAYOUT_REFERENCE, PLB_DTYPE_F64, v.data()));
+    plbpd::check(plb_download(state_, PLB_FIELD_N, PLB_LAYOUT_REFERENCE, PLB_DTYPE_F64, f.data()));
+    if (literal) plbpd::check(plb_download(state_, PLB_FIELD_NTMP, PLB_LAYOUT_REFERENCE, PLB_DTYPE_F64, t.data()));
+    plbpd::write_state_file(dump_file, X, Y, Z, kdim_, r.data(), v.data(), f.data(), literal ? t.data() : nullptr);
 }
 
+// Uploads the file's FP64 arrays to the device state (ntmp only when the file
+// holds it), then refreshes the host mirror.
 void SimData::read_state(std::string dump_file) {
     size_t X, Y, Z;
     std::tie(X, Y, Z) = dims_;
-    plbpd::read_state_file(dump_file, X, Y, Z, kdim_, rho->data(), u->data(), n->data(), ntmp->data());
-    sync_to_device();
+    const size_t cells = (X + 2) * (Y + 2) * (Z + 2);
+    std::vector<double> r(cells), v(3 * cells), f(kdim_ * cells), t(kdim_ * cells);
+    const bool has_ntmp = plbpd::read_state_file(dump_file, X, Y, Z, kdim_, r.data(), v.data(), f.data(), t.data());
+    plbpd::check(plb_upload(state_, PLB_FIELD_N, PLB_LAYOUT_REFERENCE, PLB_DTYPE_F64, f.data()));
+    if (cfg_.mode == PLB_MODE_LITERAL) {
+        plbpd::check(plb_upload(state_, PLB_FIELD_RHO, PLB_LAYOUT_REFERENCE, PLB_DTYPE_F64, r.data()));
+        plbpd::check(plb_upload(state_, PLB_FIELD_U, PLB_LAYOUT_REFERENCE, PLB_DTYPE_F64, v.data()));
+        if (has_ntmp) plbpd::check(plb_upload(state_, PLB_FIELD_NTMP, PLB_LAYOUT_REFERENCE, PLB_DTYPE_F64, t.data()));
+    }
+    sync_to_host();
 }

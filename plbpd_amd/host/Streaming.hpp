@@ -20,9 +20,6 @@ class Streaming final {
     ~Streaming();
     void operator()(SimData& simdata, std::string stream_type, bool reference = false) const;
     float get_total_time() const;
-
- private:
-    static float _time;
 };
 
 #endif  // PLBPD_AMD_HOST_STREAMING_HPP_

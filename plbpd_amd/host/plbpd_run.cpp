@@ -3,6 +3,8 @@
 //   plbpd_run [Couette2D|Couette3D|Poiseuille2D|Poiseuille3D|<scenario file>]
 //             [--fused] [--variant f64|f32|f64a32|f32s|f16s] [--steps N]
 //             [--out FILE] [--device-times]
+// print_times reports the GPU time of each reference stage over the run;
+// --device-times adds the per-stage times of the one device state.
 // A scenario file (plbpd_scenario.hpp) may set the variant, the mode, the
 // initial state and interior solids; command-line options take precedence.
 #include <cstdlib>
@@ -35,7 +37,6 @@ int main(int argc, char** argv) {
     }
     plbpd::DeviceConfig cfg;
     cfg.mode = fused ? PLB_MODE_FUSED : PLB_MODE_LITERAL;
-    cfg.device_timers = device_times;
     if (variant == "f64") cfg.variant = PLB_F64;
     else if (variant == "f32") cfg.variant = PLB_F32;
     else if (variant == "f64a32") cfg.variant = PLB_F64_A32;

@@ -1,5 +1,6 @@
 #include "plbpd_state.hpp"
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <cstdio>
@@ -54,15 +55,15 @@ bool state_files_are_hdf5() {
 #endif
 }
 
-void write_state_file(const std::string& path, size_t X, size_t Y, size_t Z, size_t Q, const real* rho,
-                      const real* u, const real* n, const real* ntmp) {
+void write_state_file(const std::string& path, size_t X, size_t Y, size_t Z, size_t Q, const double* rho,
+                      const double* u, const double* n, const double* ntmp) {
     const size_t cells = (X + 2) * (Y + 2) * (Z + 2);
     const std::vector<float> r32 = convert<float>(rho, cells);
     const std::vector<float> u32 = convert<float>(u, 3 * cells);
-    const std::vector<double> r64 = convert<double>(rho, cells);
-    const std::vector<double> u64 = convert<double>(u, 3 * cells);
-    const std::vector<double> n64 = convert<double>(n, Q * cells);
-    const std::vector<double> t64 = ntmp ? convert<double>(ntmp, Q * cells) : std::vector<double>();
+    const std::vector<double> r64(rho, rho + cells);
+    const std::vector<double> u64(u, u + 3 * cells);
+    const std::vector<double> n64(n, n + Q * cells);
+    const std::vector<double> t64 = ntmp ? std::vector<double>(ntmp, ntmp + Q * cells) : std::vector<double>();
 #ifdef PLBPD_HAVE_HDF5
     const hid_t file = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
     if (file < 0) throw std::runtime_error("cannot create " + path);
@@ -95,8 +96,8 @@ void write_state_file(const std::string& path, size_t X, size_t Y, size_t Z, siz
 #endif
 }
 
-bool read_state_file(const std::string& path, size_t X, size_t Y, size_t Z, size_t Q, real* rho, real* u,
-                     real* n, real* ntmp) {
+bool read_state_file(const std::string& path, size_t X, size_t Y, size_t Z, size_t Q, double* rho, double* u,
+                     double* n, double* ntmp) {
     const size_t cells = (X + 2) * (Y + 2) * (Z + 2);
     std::vector<double> r64(cells), u64(3 * cells), n64(Q * cells), t64;
     bool has_tmp = false;
@@ -141,11 +142,10 @@ bool read_state_file(const std::string& path, size_t X, size_t Y, size_t Z, size
     std::fclose(f);
     if (!ok) throw std::runtime_error("not a state file of this shape: " + path);
 #endif
-    for (size_t i = 0; i < cells; ++i) rho[i] = (real)r64[i];
-    for (size_t i = 0; i < 3 * cells; ++i) u[i] = (real)u64[i];
-    for (size_t i = 0; i < Q * cells; ++i) n[i] = (real)n64[i];
-    if (has_tmp && ntmp)
-        for (size_t i = 0; i < Q * cells; ++i) ntmp[i] = (real)t64[i];
+    std::copy(r64.begin(), r64.end(), rho);
+    std::copy(u64.begin(), u64.end(), u);
+    std::copy(n64.begin(), n64.end(), n);
+    if (has_tmp && ntmp) std::copy(t64.begin(), t64.end(), ntmp);
     return has_tmp;
 }
 

@@ -17,11 +17,13 @@
 
 namespace plbpd {
 
-void write_state_file(const std::string& path, size_t X, size_t Y, size_t Z, size_t Q, const real* rho,
-                      const real* u, const real* n, const real* ntmp);
+// The arrays are FP64 whatever the host mirror's type, so that a state file
+// holds the device state exactly (FP64 states included) for a resume.
+void write_state_file(const std::string& path, size_t X, size_t Y, size_t Z, size_t Q, const double* rho,
+                      const double* u, const double* n, const double* ntmp);
 // Returns true when the file held ntmp (then written to `ntmp` if non-null).
-bool read_state_file(const std::string& path, size_t X, size_t Y, size_t Z, size_t Q, real* rho, real* u,
-                     real* n, real* ntmp);
+bool read_state_file(const std::string& path, size_t X, size_t Y, size_t Z, size_t Q, double* rho, double* u,
+                     double* n, double* ntmp);
 bool state_files_are_hdf5();
 
 }  // namespace plbpd

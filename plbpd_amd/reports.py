@@ -202,6 +202,26 @@ def c5_domains(n=512, seed=5):
     return doms
 
 
+def _rank_correlations(rows):
+    """Spearman rank correlation of the per-domain velocity error with each
+    domain attribute (the steady channel velocity g H^2 / (8 nu), H the
+    fluid height between the walls, among them)."""
+    def ranks(v):
+        order = np.argsort(v, kind="stable")
+        r = np.empty(len(v))
+        r[order] = np.arange(len(v))
+        return r
+    err = ranks(np.array([r["u_l2"] for r in rows]))
+    attrs = {
+        "tau": [r["tau"] for r in rows], "g": [r["g"] for r in rows],
+        "cells": [r["x"] * r["y"] * r["z"] for r in rows], "wall_gap_z": [r["z"] - 2 for r in rows],
+        "u_max_f64": [r["u_max_f64"] for r in rows],
+        "steady_u": [r["g"] * (r["z"] - 2) ** 2 / (8 * (r["tau"] - 0.5) / 3) for r in rows],
+        "diffusion_time_over_steps": [(r["z"] - 2) ** 2 / ((r["tau"] - 0.5) / 3) for r in rows],
+    }
+    return {k: float(np.corrcoef(err, ranks(np.array(v)))[0, 1]) for k, v in attrs.items()}
+
+
 def report_c5(quick=False):
     doms = c5_domains(64 if quick else 512)
     steps = 200 if quick else 2000
@@ -228,13 +248,19 @@ def report_c5(quick=False):
     bref = P.Batch(19, "f64", doms, bc)
     bref.initialize(1.0, (0, 0, 0), init)
     bref.step(steps, steps_per_graph=20)
-    errs = []
+    errs, per = [], []
     for d in range(len(doms)):
         _, u = b.moments(d)
         _, ur = bref.moments(d)
         fl = (slice(1, -1),)
-        errs.append(_l2(u[fl], ur[fl]))
+        e = _l2(u[fl], ur[fl])
+        errs.append(e)
+        dm = doms[d]
+        per.append({"domain": d, "x": dm["x"], "y": dm["y"], "z": dm["z"], "tau": dm["tau"],
+                    "g": dm["force"][0], "u_max_f64": float(np.abs(ur[fl][..., 0]).max()), "u_l2": e})
     out["per_domain_u_l2_vs_f64"] = {"max": max(errs), "median": float(np.median(errs)), "min": min(errs)}
+    out["largest_errors"] = sorted(per, key=lambda r: -r["u_l2"])[:12]
+    out["error_rank_correlation"] = _rank_correlations(per)
     bref.close()
     # the same steps with one launch per domain
     b.initialize(1.0, (0, 0, 0), init)

@@ -7,6 +7,7 @@
 //                     and fused mode; state files; error types of device calls
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -45,7 +46,6 @@ int orc_lit_init_f32(int q, int X, int Y, int Z, double rho0, const double* u0, 
                      const orc_bc* bc, float* rho, float* u, float* n, float* ntmp);
 int orc_lit_run_f32(int q, int X, int Y, int Z, double tau, const double* F, const orc_bc* bc, int nsteps,
                     float* rho, float* u, float* bufA, float* bufB);
-void orc_lit_set_avx2(int on);
 int orc_cor_init(int variant, int q, int X, int Y, int Z, double rho0, const double* u0, const orc_init* in,
                  void* f);
 int orc_cor_run(int variant, int q, int X, int Y, int Z, double tau, const double* F, const orc_bc* bc,
@@ -76,6 +76,15 @@ bool throws(const std::function<void()>& fn) {
         return false;
     }
     return false;
+}
+
+// Oracle values (float or double) equal the host mirror's, each rounded once
+// to the host field type plbpd::real.
+template <class T>
+bool same_as_host(const std::vector<T>& orc, const plbpd::real* host) {
+    for (size_t i = 0; i < orc.size(); ++i)
+        if ((plbpd::real)orc[i] != host[i]) return false;
+    return true;
 }
 
 // ------------------------------------------------------------------ CPU ---
@@ -303,15 +312,17 @@ void test_literal_march_matches_oracle() {
     orc_lit_init_f64(19, X, Y, Z, 1.0, u0, &in, &ob, rho.data(), u.data(), a.data(), b.data());
     const int which = orc_lit_run_f64(19, X, Y, Z, tau, F.data(), &ob, steps, rho.data(), u.data(), a.data(), b.data());
     const std::vector<double>& n = which == 0 ? a : b;
-    EXPECT(std::memcmp(rho.data(), sim.rho->data(), cells * sizeof(double)) == 0);
-    EXPECT(std::memcmp(u.data(), sim.u->data(), 3 * cells * sizeof(double)) == 0);
-    EXPECT(std::memcmp(n.data(), sim.n->data(), 19 * cells * sizeof(double)) == 0);
+    // the host fields hold the FP64 device values, rounded once when they
+    // are float (the reference's field type)
+    EXPECT(same_as_host(rho, sim.rho->data()));
+    EXPECT(same_as_host(u, sim.u->data()));
+    EXPECT(same_as_host(n, sim.n->data()));
 }
 
-// An FP32 literal state marched through the drop-in classes: CollisionSRT is
-// called with reference = false, the reference's default loop, so the
-// collision follows the AVX2 kernel of its default build.
-void test_literal_fp32_march_matches_avx2_oracle() {
+// An FP32 literal state marched through the drop-in classes (CollisionSRT
+// called with reference = false, the reference's default loop): the
+// collision in the scalar kernel's float/double promotions.
+void test_literal_fp32_march_matches_oracle() {
     const int X = 12, Y = 8, Z = 10, steps = 30;
     const double tau = 0.7;
     const std::array<double, 3> F = {1e-5, 0.0, 0.0};
@@ -331,17 +342,74 @@ void test_literal_fp32_march_matches_avx2_oracle() {
     const orc_init in = {PLB_INIT_PERTURBED, 77, 1e-3};
     const double u0[3] = {0, 0, 0};
     orc_lit_init_f32(19, X, Y, Z, 1.0, u0, &in, &ob, rho.data(), u.data(), a.data(), b.data());
-    orc_lit_set_avx2(1);
     const int which = orc_lit_run_f32(19, X, Y, Z, tau, F.data(), &ob, steps, rho.data(), u.data(), a.data(),
                                       b.data());
-    orc_lit_set_avx2(0);
     const std::vector<float>& n = which == 0 ? a : b;
-    bool same = true;
-    for (size_t i = 0; i < cells; ++i) same &= (double)rho[i] == (double)sim.rho->data()[i];
-    for (size_t i = 0; i < 3 * cells; ++i) same &= (double)u[i] == (double)sim.u->data()[i];
-    for (size_t i = 0; i < 19 * cells; ++i) same &= (double)n[i] == (double)sim.n->data()[i];
-    if (!same) std::fprintf(stderr, "FP32 literal march differs from the AVX2-arithmetic oracle\n");
+    const bool same = same_as_host(rho, sim.rho->data()) && same_as_host(u, sim.u->data()) &&
+                      same_as_host(n, sim.n->data());
+    if (!same) std::fprintf(stderr, "FP32 literal march differs from the oracle\n");
     EXPECT(same);
+}
+
+// The reference-interface timers (Collision, Streaming, Boundary, CalcMoments
+// getters, printed by PLBPD::print_times) report GPU time: over a run of the
+// reference's staged calls they grow by exactly the state's HIP-event stage
+// times (plb_stage_times), and the collisions' GPU time exceeds the host time
+// spent issuing every call of the loop, which is all that timing the calls
+// on the host would measure.
+void test_timers_report_gpu_time() {
+    const int N = 128, steps = 20;
+    Setup s = couette(N, N, N, 0.7, {1e-5, 0.0, 0.0}, 0.02f);
+    plbpd::DeviceConfig cfg;  // HIP-event timers on by default
+    EXPECT(cfg.device_timers);
+    cfg.variant = PLB_F64;
+    SimData sim(s.domain->get_dimensions(), 19, cfg);
+    PLBPD p(s.model.get(), s.domain.get(), s.boundary.get());
+    p.initialize(sim);
+    LBDynamics dyn(s.model.get(), s.domain.get());
+    CalcMoments moments(s.model.get());
+    const Boundary& bnd = *s.boundary;
+    auto getters = [&] {
+        return std::array<double, 7>{dyn.get_time_collide(),       dyn.get_time_stream(),
+                                     bnd.get_time_noslip(),        bnd.get_time_periodicity(),
+                                     moments.get_total_time(),     bnd.get_time_velocity_reset(),
+                                     bnd.get_time_density_reset()};
+    };
+    double dev0[PLB_NSTAGES];
+    plbpd::check(plb_stage_times(sim.device(), dev0));
+    const std::array<double, 7> before = getters();
+    double host_issue = 0.0;
+    for (int i = 0; i < steps; ++i) {
+        const auto t0 = std::chrono::steady_clock::now();
+        dyn.collide(sim);
+        dyn.stream(sim, "push");
+        bnd.apply_noslip(sim);
+        bnd.apply_periodicity(sim);
+        moments(sim);
+        bnd.reset(sim);
+        host_issue += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    const std::array<double, 7> after = getters();
+    double dev1[PLB_NSTAGES];
+    plbpd::check(plb_stage_times(sim.device(), dev1));
+    // getter order: collide, stream (+ fused), noslip, periodic, moments, resets
+    const int stage[7] = {PLB_STAGE_COLLIDE, PLB_STAGE_STREAM, PLB_STAGE_NOSLIP, PLB_STAGE_PERIODIC,
+                          PLB_STAGE_MOMENTS, PLB_STAGE_RESET_VELOCITY, PLB_STAGE_RESET_DENSITY};
+    for (int i = 0; i < 7; ++i) {
+        double dev = (dev1[stage[i]] - dev0[stage[i]]) * 1e-3;
+        if (i == 1) dev += (dev1[PLB_STAGE_FUSED] - dev0[PLB_STAGE_FUSED]) * 1e-3;
+        const double got = after[i] - before[i];
+        // the getters return float seconds of the process-wide sums
+        const double tol = 1e-6 * (1.0 + std::fabs(after[i])) * 4;
+        if (std::fabs(got - dev) > tol)
+            std::fprintf(stderr, "timer %d: getter delta %.9g s, device %.9g s\n", i, got, dev);
+        EXPECT(std::fabs(got - dev) <= tol);
+    }
+    const double collide_gpu = (dev1[PLB_STAGE_COLLIDE] - dev0[PLB_STAGE_COLLIDE]) * 1e-3;
+    if (!(collide_gpu > host_issue))
+        std::fprintf(stderr, "collide GPU %.6f s vs host issue %.6f s\n", collide_gpu, host_issue);
+    EXPECT(collide_gpu > host_issue);
+    EXPECT(after[0] > before[0] && after[2] > before[2] && after[3] > before[3]);
 }
 
 void test_fused_march_matches_oracle(int variant, float lid) {
@@ -376,8 +444,8 @@ void test_fused_march_matches_oracle(int variant, float lid) {
             for (int x = 0; x < X; ++x) {
                 const size_t c = ((size_t)z * Y + y) * X + x;
                 if (flags[c]) continue;
-                same &= sim.rho->at(z + 1, y + 1, x + 1) == rho[c];
-                for (int i = 0; i < 3; ++i) same &= sim.u->at(z + 1, y + 1, x + 1, i) == u[3 * c + i];
+                same &= sim.rho->at(z + 1, y + 1, x + 1) == (plbpd::real)rho[c];
+                for (int i = 0; i < 3; ++i) same &= sim.u->at(z + 1, y + 1, x + 1, i) == (plbpd::real)u[3 * c + i];
             }
     if (!same) std::fprintf(stderr, "fused march differs from the oracle: variant %d, lid %g\n", variant, lid);
     EXPECT(same);
@@ -422,8 +490,8 @@ void test_scenario_march_matches_oracle() {
                 const size_t c = ((size_t)z * Y + y) * X + x;
                 if (flags[c]) continue;
                 ++fluid;
-                same &= sim.rho->at(z + 1, y + 1, x + 1) == rho[c];
-                for (int i = 0; i < 3; ++i) same &= sim.u->at(z + 1, y + 1, x + 1, i) == u[3 * c + i];
+                same &= sim.rho->at(z + 1, y + 1, x + 1) == (plbpd::real)rho[c];
+                for (int i = 0; i < 3; ++i) same &= sim.u->at(z + 1, y + 1, x + 1, i) == (plbpd::real)u[3 * c + i];
             }
     EXPECT(fluid > 1500);
     EXPECT(same);
@@ -458,10 +526,13 @@ void test_device_errors() {
     EXPECT(throws<std::invalid_argument>([&] { SimData z(std::make_tuple(0, 4, 4), 19, cfg); }));
 }
 
+// An FP64 state written and read back resumes exactly, whatever the host
+// fields' type: the files hold FP64 downloads of the device state.
 void test_state_file_resume() {
     const int X = 10, Y = 6, Z = 8;
     Setup s = couette(X, Y, Z, 0.75, {1e-5, 0, 0}, 0.02f);
     plbpd::DeviceConfig cfg;
+    cfg.variant = PLB_F64;
     SimData a(s.domain->get_dimensions(), 19, cfg), b(s.domain->get_dimensions(), 19, cfg);
     PLBPD p(s.model.get(), s.domain.get(), s.boundary.get());
     p.initialize(a);
@@ -474,7 +545,11 @@ void test_state_file_resume() {
     p.march_in_time(10, b);
     a.sync_to_host();
     b.sync_to_host();
-    EXPECT(std::memcmp(a.n->data(), b.n->data(), a.n->size() * sizeof(double)) == 0);
+    EXPECT(std::memcmp(a.n->data(), b.n->data(), a.n->size() * sizeof(plbpd::real)) == 0);
+    std::array<double, 4> ga{}, gb{};
+    plbpd::check(plb_global_moments(a.device(), ga.data()));
+    plbpd::check(plb_global_moments(b.device(), gb.data()));
+    EXPECT(ga == gb);
     std::remove(path.c_str());
 }
 
@@ -517,8 +592,8 @@ void test_move_keeps_state() {
     p.march_in_time(20, ref);
     b.sync_to_host();
     ref.sync_to_host();
-    EXPECT(std::memcmp(b.n->data(), ref.n->data(), b.n->size() * sizeof(double)) == 0);
-    EXPECT(std::memcmp(b.u->data(), ref.u->data(), b.u->size() * sizeof(double)) == 0);
+    EXPECT(std::memcmp(b.n->data(), ref.n->data(), b.n->size() * sizeof(plbpd::real)) == 0);
+    EXPECT(std::memcmp(b.u->data(), ref.u->data(), b.u->size() * sizeof(plbpd::real)) == 0);
 }
 
 }  // namespace
@@ -536,7 +611,8 @@ int main(int argc, char** argv) {
     }
     if (what == "gpu" || what == "all") {
         test_literal_march_matches_oracle();
-        test_literal_fp32_march_matches_avx2_oracle();
+        test_literal_fp32_march_matches_oracle();
+        test_timers_report_gpu_time();
         for (int v : {PLB_F64, PLB_F32S, PLB_F16S}) test_fused_march_matches_oracle(v, 0.0f);
         for (int v : {PLB_F64, PLB_F32S}) test_fused_march_matches_oracle(v, 0.03f);  // moving lid
         test_scenario_march_matches_oracle();

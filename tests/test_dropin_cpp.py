@@ -7,21 +7,26 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_dropin")
+BIN64 = os.path.join(ROOT, "tests", "cpp", "_build", "test_dropin_f64")
 RUN = os.path.join(ROOT, "plbpd_amd", "lib", "plbpd_run")
 
 needs_bin = pytest.mark.skipif(not os.path.exists(BIN), reason="tests/cpp not built (run __graft_entry__.build())")
 
 
 @needs_bin
-def test_dropin_reference_known_answers():
-    r = subprocess.run([BIN, "cpu"], capture_output=True, text=True, timeout=120)
+@pytest.mark.parametrize("binary", [BIN, BIN64], ids=["float_fields", "double_fields"])
+def test_dropin_reference_known_answers(binary):
+    r = subprocess.run([binary, "cpu"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 
 
 @needs_bin
 @pytest.mark.gpu
-def test_dropin_march_in_time_matches_oracle():
-    r = subprocess.run([BIN, "gpu"], capture_output=True, text=True, timeout=300)
+@pytest.mark.parametrize("binary", [BIN, BIN64], ids=["float_fields", "double_fields"])
+def test_dropin_march_in_time_matches_oracle(binary):
+    """The drop-in classes with the reference's float host fields (the
+    default build) and with double fields (-DPLBPD_REAL=double)."""
+    r = subprocess.run([binary, "gpu"], capture_output=True, text=True, timeout=300)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr
 
@@ -30,8 +35,9 @@ MAIN14 = os.path.join(ROOT, "tests", "cpp", "_build", "main_cxx14")
 
 
 def test_reference_main_compiles_as_cxx14():
-    """The reference's main() call sequence compiles as C++14 (the reference's
-    standard, src/CMakeLists.txt:6) against the drop-in headers."""
+    """A caller of the reference's public interface (the call sequence of
+    main.cpp:13-31) compiles as C++14 (the reference's standard,
+    src/CMakeLists.txt:6) against the drop-in headers."""
     src = os.path.join(ROOT, "tests", "cpp", "main_cxx14.cpp")
     r = subprocess.run(["g++", "-std=c++14", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", src,
                         "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "plbpd_amd", "host")],
@@ -54,12 +60,16 @@ def test_reference_interface_signatures(real):
 @pytest.mark.gpu
 def test_reference_main_runs(tmp_path):
     """The same program on the GPU: Couette3D (200x50x50 D3Q19, 200 steps,
-    literal FP64), writing the initial and the final state."""
+    literal, the default build's float fields and FP32 device state), writing
+    the initial and the final state and printing the class timers, whose
+    collide line reports GPU time."""
     r = subprocess.run([MAIN14], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Time taken by timeloop" in r.stdout
-    init, final = (tmp_path / "init_state.h5").read_bytes(), (tmp_path / "final_state.h5").read_bytes()
+    init, final = (tmp_path / "Couette3D_begin.h5").read_bytes(), (tmp_path / "Couette3D_end.h5").read_bytes()
     assert len(init) == len(final) > 0 and init != final
+    collide = [l for l in r.stdout.splitlines() if l.startswith("-collide:")]
+    assert collide and float(collide[0].split(":")[1].strip().rstrip("s")) > 0, r.stdout
 
 
 @pytest.mark.gpu

@@ -135,3 +135,27 @@ def test_batch_autotune_and_cells():
     assert total == sum(d["x"] * d["y"] * d["z"] for d in doms)
     assert fluid == sum(d["x"] * d["y"] * (d["z"] - 2) for d in doms)
     b.close()
+
+
+def test_autotune_advances_the_documented_steps():
+    """plb_batch_autotune runs real steps: every candidate advances the batch
+    by 2 + (steps rounded up to even).  After it, the batch equals a twin
+    stepped that many times per reported candidate, bit for bit."""
+    print("\n[case] batch autotune advance", flush=True)
+    doms = domains(5, 3)
+    _, pb = channel()
+    init = (P.INIT_PERTURBED, 9, 1e-3)
+    a = P.Batch(19, "f32s", doms, pb)
+    b = P.Batch(19, "f32s", doms, pb)
+    for x in (a, b):
+        x.initialize(1.0, (0, 0, 0), init)
+    steps = 3
+    res = a.autotune(steps=steps, max_results=128)
+    assert len(res) == 60
+    b.step(len(res) * (2 + steps + (steps & 1)))
+    for d in range(len(doms)):
+        ra, ua = a.moments(d)
+        rb, ub = b.moments(d)
+        assert np.array_equal(ra, rb) and np.array_equal(ua, ub), d
+    a.close()
+    b.close()

@@ -187,51 +187,6 @@ def test_literal_tiny_domains_all_walls_bitexact(orc, q, X, Y, Z, staged):
         lit_compare(st, o, "after 5 steps")
 
 
-@pytest.mark.parametrize("q,X,Y,Z", LITERAL_CASES)
-@pytest.mark.parametrize("path", ["step", "staged_reads", "staged"])
-def test_literal_fp32_avx2_arithmetic_bitexact(orc, q, X, Y, Z, path):
-    """FP32 literal mode with PLB_TUNE_LITERAL_AVX2: the collision in the
-    arithmetic of the reference's default AVX2 build (fused multiply-adds for
-    the first 8 * floor(q / 8) directions), against the oracle's restatement
-    of it, which tests/test_oracle_avx2.py pins.  Through plb_step, through
-    the staged calls with a read after every stage, and through staged calls
-    the library folds."""
-    print(f"\n[case] literal f32 avx2 q={q} {X}x{Y}x{Z} {path}", flush=True)
-    types = couette_types(q)
-    ob, pb = both_bc(types, {UP: (0.04, 0.0, 0.0), DOWN: (0.0, 0.0, 0.0)})
-    tau, F = 0.62, (2e-4, -1e-5 if q != 9 else 0.0, 1e-5)
-    o = orc.lit_state(q, X, Y, Z, np.float32)
-    orc.lit_init(o, 1.0, (0.02, 0.0, 0.0), (P.INIT_PERTURBED, 17, 1e-2), ob)
-    orc.lit_set_avx2(True)
-    try:
-        with P.State(q, X, Y, Z, "f32", P.LITERAL) as st:
-            st.set_tuning(P.capi.TUNE_LITERAL_AVX2, 1)
-            st.initialize(1.0, (0.02, 0.0, 0.0), (P.INIT_PERTURBED, 17, 1e-2), pb)
-            if path == "step":
-                orc.lit_run(o, tau, F, ob, 12)
-                st.configure(tau, F, pb)
-                st.step(12)
-            else:
-                for step in range(4):
-                    orc.lit_collide(o, tau, F)
-                    st.collide(tau, F)
-                    if path == "staged_reads":
-                        lit_compare(st, o, f"step {step} collide")
-                    orc.lit_push(o)
-                    assert orc.lit_noslip(o, ob) == 0
-                    orc.lit_periodic(o, ob)
-                    orc.lit_moments(o)
-                    orc.lit_reset(o, ob)
-                    st.stream("push")
-                    st.apply_noslip(pb)
-                    st.apply_periodicity(pb)
-                    st.calc_moments()
-                    st.reset(pb)
-            lit_compare(st, o, f"{path} end")
-    finally:
-        orc.lit_set_avx2(False)
-
-
 def test_literal_deferred_collision_runs_before_other_calls(orc):
     """A literal collision waits only for a directly following push.  Two
     collisions in a row both run; a stream request the literal mode refuses
@@ -266,6 +221,56 @@ def test_literal_deferred_collision_runs_before_other_calls(orc):
         st.set_tuning(P.capi.TUNE_LITERAL_FUSE, 0)
         st.stream("push")
         lit_compare(st, o, "knob off with a queued collision")
+
+
+@pytest.mark.parametrize("q,X,Y,Z", [(19, 9, 7, 6), (27, 7, 6, 8)])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_literal_ntmp_after_folded_push(orc, q, X, Y, Z, prec):
+    """The collide+push kernel stores in-place values on the outer layer
+    only; ntmp elsewhere is rebuilt from n before anything reads it or
+    changes n.  After folded steps: a collision that runs alone (a read
+    follows it), a push alone, and an upload of n must each leave ntmp as the
+    reference's separate stages do."""
+    print(f"\n[case] literal ntmp after folded push q={q} {X}x{Y}x{Z} {prec}", flush=True)
+    dt = np.float64 if prec == "f64" else np.float32
+    ob, pb = both_bc(couette_types(q), {UP: (0.04, 0.01, 0.0), DOWN: (0.0, 0.0, 0.0)})
+    tau, F = 0.66, (1e-4, -2e-5, 1e-5)
+    o = orc.lit_state(q, X, Y, Z, dt)
+    orc.lit_init(o, 1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 8, 1e-2), ob)
+    with P.State(q, X, Y, Z, prec, P.LITERAL) as st:
+        st.initialize(1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 8, 1e-2), pb)
+        for _ in range(2):  # folded: collide, push, and the step before's moments and resets
+            orc.lit_collide(o, tau, F)
+            orc.lit_push(o)
+            assert orc.lit_noslip(o, ob) == 0
+            orc.lit_periodic(o, ob)
+            orc.lit_moments(o)
+            orc.lit_reset(o, ob)
+            st.collide(tau, F)
+            st.stream("push")
+            st.apply_noslip(pb)
+            st.apply_periodicity(pb)
+            st.calc_moments()
+            st.reset(pb)
+        orc.lit_collide(o, tau, F)
+        st.collide(tau, F)
+        lit_compare(st, o, "collision run alone after a folded push")
+        orc.lit_push(o)
+        st.stream("push")
+        lit_compare(st, o, "push alone")
+        orc.lit_collide(o, tau, F)
+        orc.lit_push(o)
+        st.collide(tau, F)
+        st.stream("push")
+        new_n = (o["n"] * (1 + 1e-3 * np.random.default_rng(2).standard_normal(o["n"].shape))).astype(dt)
+        o["n"][...] = new_n
+        st.upload(P.FIELD_N, new_n)
+        lit_compare(st, o, "n uploaded after a folded push")
+        orc.lit_collide(o, tau, F)
+        orc.lit_push(o)
+        st.collide(tau, F)
+        st.stream("push")
+        lit_compare(st, o, "folded push after the upload")
 
 
 @pytest.mark.parametrize("q,X,Y,Z", LITERAL_CASES)
@@ -433,6 +438,40 @@ def test_default_widths_around_solids(orc, q, X, Y, Z, variant):
         assert np.array_equal(raw.view(np.uint16)[fluid], o["f"].view(np.uint16)[fluid])
     else:
         assert np.array_equal(raw[fluid], o["f"][fluid])
+
+
+@pytest.mark.parametrize("q,X,Y,Z,variant,vec,block", [
+    (19, 256, 6, 5, "f64", -1, 128),   # automatic width: 2 cells per lane, two 128-cell segments
+    (27, 256, 4, 4, "f64", -1, 128),   # D3Q27 around solids: 2 cells per lane
+    (19, 512, 4, 3, "f32s", -1, 256),  # 4 cells per lane, two 256-cell segments
+    (19, 256, 6, 5, "f32s", 1, 128),   # four 64-cell segments, two of them all solid
+    (27, 384, 4, 3, "f32", 2, 128),    # three 128-cell segments
+])
+def test_row_pull_all_solid_segment(orc, q, X, Y, Z, variant, vec, block):
+    """A solid block covers whole wavefront segments of some rows, so those
+    wavefronts have no fluid cell.  Their neighbours in the row still pull
+    the diagonal populations those wavefronts loaded from the fluid rows
+    around the block: the row kernel must publish segment edges for every
+    wavefront of an existing row, fluid or not."""
+    print(f"\n[case] all-solid segment q={q} {X}x{Y}x{Z} {variant} vec={vec} block={block}", flush=True)
+    ob, pb = fused_bc(q, True, lid=(0.02, 0.0, 0.0))
+    mask = (np.random.default_rng(23).random((Z, Y, X)) < 0.1).astype(np.uint8)
+    mask[:, 2:4, 0:block] = 1
+    init = (P.INIT_PERTURBED, 41, 1e-3)
+    tau, F = 0.66, (1e-5, 2e-6, 0)
+    o = orc.cor_state(variant, q, X, Y, Z)
+    orc.cor_init(o, 1.0, (0.01, 0, 0), init)
+    orc.cor_run(o, tau, F, ob, 10, mask)
+    with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+        st.set_tuning(P.capi.TUNE_SCHEME, SCHEMES["pull"])
+        st.set_tuning(P.capi.TUNE_VECTOR, vec)
+        st.set_boundary(pb, mask)
+        st.initialize(1.0, (0.01, 0, 0), init, pb)
+        st.configure(tau, F, pb)
+        st.step(10)
+        raw = st.download_raw()
+    fluid = orc.cor_flags(q, X, Y, Z, ob, mask) == 0
+    assert np.array_equal(raw[fluid], o["f"][fluid])
 
 
 @pytest.mark.parametrize("variant", ["f64", "f32", "f32s", "f16s"])
