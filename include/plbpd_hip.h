@@ -186,6 +186,19 @@ int plb_collide_srt(plb_state* s, double tau, const double force[3]);
 int plb_stream(plb_state* s, const char* stream_type);
 /* Boundary::apply_noslip (Boundary.cpp:101-111, 189-221). */
 int plb_apply_noslip(plb_state* s, const plb_boundary* bc);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  plb_apply_noslip on
+ * a literal state (the same values) with its staging in a static device
+ * buffer of 16 MiB that every GPU the library runs on holds until the
+ * process ends, instead of the state's scratch buffer, which plb_apply_noslip
+ * allocates on first use.  Calls no hipMalloc or hipFree and creates no
+ * event: the calls deferred before it and the pass itself are not timed.
+ * Each no-slip face needs (face cells) * (q - 1) * sizeof(value) bytes; a
+ * larger face -> PLB_ERR_INVALID_ARGUMENT before anything runs.  Fused
+ * states -> PLB_ERR_LOGIC.  Every call on one GPU shares the buffer, so calls
+ * that overlap on one GPU (states on different streams, other threads) race;
+ * nothing in it needs to start at zero. */
+int plb_apply_noslip_noalloc(plb_state* s, const plb_boundary* bc);
 /* Boundary::apply_periodicity (Boundary.cpp:91-99, 223-281). */
 int plb_apply_periodicity(plb_state* s, const plb_boundary* bc);
 /* CalcMoments::operator() (CalcMoments.hpp:46-73).  Literal mode with

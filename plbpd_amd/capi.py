@@ -138,8 +138,8 @@ def lib():
     L.plb_set_boundary.argtypes = [vp, C.POINTER(BoundaryC), vp]
     L.plb_collide_srt.argtypes = [vp, d, vp]
     L.plb_stream.argtypes = [vp, C.c_char_p]
-    for n in ("plb_apply_noslip", "plb_apply_periodicity", "plb_reset", "plb_reset_velocity",
-              "plb_reset_density"):
+    for n in ("plb_apply_noslip", "plb_apply_noslip_noalloc", "plb_apply_periodicity", "plb_reset",
+              "plb_reset_velocity", "plb_reset_density"):
         getattr(L, n).argtypes = [vp, C.POINTER(BoundaryC)]
     L.plb_calc_moments.argtypes = [vp]
     L.plb_configure.argtypes = [vp, d, vp, C.POINTER(BoundaryC)]

@@ -509,6 +509,22 @@ int plb_apply_noslip(plb_state* s, const plb_boundary* bc) {
     return ensure_fused_bc(s, bc, &s->mask_host);
 }
 
+int plb_apply_noslip_noalloc(plb_state* s, const plb_boundary* bc) {
+    int rc = enter_keep(s);
+    if (rc) return rc;
+    if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
+    if (s->mode != PLB_MODE_LITERAL)
+        return fail(PLB_ERR_LOGIC, "plb_apply_noslip_noalloc is for literal states (the fused step folds no-slip in)");
+    // The deferred calls and this pass run untimed: a timer may need a new
+    // event, and creating one can allocate.
+    const bool timers = s->timers_on;
+    s->timers_on = false;
+    rc = lit_flush(s);
+    if (rc == PLB_OK) rc = lit_noslip_noalloc(s, bc);
+    s->timers_on = timers;
+    return rc;
+}
+
 int plb_apply_periodicity(plb_state* s, const plb_boundary* bc) {
     int rc = enter(s);
     if (rc) return rc;

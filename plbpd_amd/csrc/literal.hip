@@ -7,6 +7,8 @@
 // R = double (FP64 contract) or float (the reference's own rounding, with its
 // float/double promotions).  Compiled with -ffp-contract=off: every
 // expression below is evaluated in the order written, as in the reference.
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "state.hpp"
@@ -663,7 +665,18 @@ int lit_push(plb_state* s) {
     return rc;
 }
 
-int lit_noslip(plb_state* s, const plb_boundary* bc) {
+// Staging of plb_apply_noslip_noalloc: one face's gathered values, held by
+// every GPU the library runs on until the process ends.  Host code uses the
+// device address hipGetSymbolAddress returns for it (noalloc_stage), never
+// the array's name.
+constexpr size_t kNoAllocStageBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char g_noalloc_stage[kNoAllocStageBytes];
+
+namespace {
+// The no-slip pass with a given staging buffer (stage_bytes of device
+// memory), or with the state's scratch buffer, grown as needed, when stage is
+// null.
+int noslip_with(plb_state* s, const plb_boundary* bc, void* stage, size_t stage_bytes) {
     const LitDims g = dims_of(s);
     for (int f = 0; f < 6; ++f) {
         if (bc->type[f] != kNoslip) continue;
@@ -674,23 +687,69 @@ int lit_noslip(plb_state* s, const plb_boundary* bc) {
             constexpr int Q = decltype(qc)::value;
             using R = decltype(rr);
             const uint64_t total = b.count() * (Q - 1);
-            int e = ensure_scratch(s, total * sizeof(R));
-            if (e) return e;
+            void* buf = stage;
+            if (!buf) {
+                const int e = ensure_scratch(s, total * sizeof(R));
+                if (e) return e;
+                buf = s->scratch;
+            }
             const R cs2inv = (R)1 / (R)(1.0 / 3.0);
             hipLaunchKernelGGL((k_noslip_gather<Q, R>), dim3(blocks_for(total, kThreads)),
                                dim3(kThreads), 0, s->stream, g, b, (const R*)s->buf[s->cur],
                                (const R*)s->rho, (R)bc->vel[f][0], (R)bc->vel[f][1],
-                               (R)bc->vel[f][2], cs2inv, (R*)s->scratch, total);
+                               (R)bc->vel[f][2], cs2inv, (R*)buf, total);
             PLB_HIP_TRY(hipGetLastError());
             hipLaunchKernelGGL((k_noslip_scatter<Q, R>), dim3(blocks_for(total, kThreads)),
                                dim3(kThreads), 0, s->stream, g, b, (R*)s->buf[s->cur],
-                               (const R*)s->scratch, total);
+                               (const R*)buf, total);
             PLB_HIP_TRY(hipGetLastError());
             return PLB_OK;
         });
         if (rc) return rc;
     }
+    (void)stage_bytes;
     return PLB_OK;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices];
+
+// The device address of g_noalloc_stage on the current device (s->device),
+// looked up on the first call on that device.
+int noalloc_stage(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(PLB_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_noalloc_stage));
+        if (e != hipSuccess || !p)
+            return fail(PLB_ERR_HIP, std::string("no-allocation staging: ") + hipGetErrorString(e));
+        g_noalloc_ptr[device] = p;
+    }
+    *out = g_noalloc_ptr[device];
+    return PLB_OK;
+}
+}  // namespace
+
+int lit_noslip(plb_state* s, const plb_boundary* bc) { return noslip_with(s, bc, nullptr, 0); }
+
+int lit_noslip_noalloc(plb_state* s, const plb_boundary* bc) {
+    // every face must fit the staging before anything runs
+    const size_t rbytes = s->variant == kF64 ? sizeof(double) : sizeof(float);
+    for (int f = 0; f < 6; ++f) {
+        if (bc->type[f] != kNoslip) continue;
+        const uint64_t need = face_box(f, s->X, s->Y, s->Z).count() * (uint64_t)(s->q - 1) * rbytes;
+        if (need > kNoAllocStageBytes)
+            return fail(PLB_ERR_INVALID_ARGUMENT,
+                        "no-slip face of " + std::to_string(need) + " bytes exceeds the " +
+                            std::to_string(kNoAllocStageBytes >> 20) +
+                            " MiB no-allocation staging; use plb_apply_noslip");
+    }
+    void* stage = nullptr;
+    const int rc = noalloc_stage(s->device, &stage);
+    if (rc) return rc;
+    return noslip_with(s, bc, stage, kNoAllocStageBytes);
 }
 
 int lit_periodic(plb_state* s, const plb_boundary* bc) {

@@ -126,6 +126,8 @@ int lit_collide_push(plb_state* s, double tau, const double F[3]);
 int lit_moments_collide_push(plb_state* s, double tau, const double F[3], const plb_boundary* bc_vel,
                              const plb_boundary* bc_rho);
 int lit_noslip(plb_state* s, const plb_boundary* bc);
+// The same pass staged in the library's static device buffer: no allocation.
+int lit_noslip_noalloc(plb_state* s, const plb_boundary* bc);
 int lit_periodic(plb_state* s, const plb_boundary* bc);
 int lit_moments(plb_state* s);
 int lit_reset_velocity(plb_state* s, const plb_boundary* bc);
