@@ -49,6 +49,12 @@ extern "C" {
  * bounce-back folded in through a per-cell flag (no stale reads). */
 #define PLB_MODE_LITERAL 0
 #define PLB_MODE_FUSED 1
+/* FUSED semantics with one population array instead of two (the AA pattern:
+ * steps alternate between an in-place local step and an in-place
+ * gather/scatter step): half the population memory, the same numbers.
+ * Reading the state after an odd number of steps gathers it into a
+ * temporary second array first.  Its scheme is PLB_SCHEME_AA. */
+#define PLB_MODE_FUSED_AA 2
 
 /* Face order of Boundary.cpp:17-18. */
 #define PLB_EAST 0
@@ -273,6 +279,7 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
 #define PLB_SCHEME_AUTO 0   /*   pull (default; moving walls included)         */
 #define PLB_SCHEME_PUSH 1   /*   collide, then scatter to neighbours            */
 #define PLB_SCHEME_PULL 2   /*   gather from neighbours, then collide           */
+#define PLB_SCHEME_AA 3     /*   one array (PLB_MODE_FUSED_AA states only)      */
 #define PLB_TUNE_TILE_ORDER 4 /* rows in B x B blocks of the y-z plane (B a power */
                               /* of two dividing the row counts; default 0 =      */
                               /* linear)                                          */

@@ -6,7 +6,7 @@ in plbpd_amd/host/.  This module is the ctypes binding used by tests and
 bench.py.
 """
 from .capi import (  # noqa: F401
-    F64, F32, F64_A32, F32S, F16S, LITERAL, FUSED,
+    F64, F32, F64_A32, F32S, F16S, LITERAL, FUSED, FUSED_AA,
     EAST, WEST, NORTH, SOUTH, UP, DOWN, UNSET, PERIODIC, NOSLIP,
     FIELD_RHO, FIELD_U, FIELD_N, FIELD_NTMP, LAYOUT_REFERENCE, LAYOUT_COMPACT,
     INIT_UNIFORM, INIT_PERTURBED, INIT_TAYLOR_GREEN, STORAGE_BYTES,

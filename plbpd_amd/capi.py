@@ -22,7 +22,7 @@ F64, F32, F64_A32, F32S, F16S = 0, 1, 2, 3, 4
 VARIANT_NAMES = {"f64": F64, "f32": F32, "f64a32": F64_A32, "f32s": F32S, "f16s": F16S}
 STORAGE_BYTES = {F64: 8, F32: 4, F64_A32: 8, F32S: 4, F16S: 2}
 STORAGE_DTYPE = {F64: np.float64, F32: np.float32, F64_A32: np.float64, F32S: np.float32, F16S: np.uint16}
-LITERAL, FUSED = 0, 1
+LITERAL, FUSED, FUSED_AA = 0, 1, 2
 EAST, WEST, NORTH, SOUTH, UP, DOWN = range(6)
 UNSET, PERIODIC, NOSLIP = 0, 1, 2
 FIELD_RHO, FIELD_U, FIELD_N, FIELD_NTMP = range(4)
@@ -31,7 +31,7 @@ DTYPE_F64, DTYPE_F32 = 0, 1
 INIT_UNIFORM, INIT_PERTURBED, INIT_TAYLOR_GREEN = 0, 1, 2
 (TUNE_BLOCK_X, TUNE_NT_LOAD, TUNE_NT_STORE, TUNE_SCHEME, TUNE_TILE_ORDER, TUNE_VECTOR, TUNE_XCD, TUNE_INDEX64,
  TUNE_LITERAL_FUSE) = range(9)
-SCHEME_AUTO, SCHEME_PUSH, SCHEME_PULL = 0, 1, 2
+SCHEME_AUTO, SCHEME_PUSH, SCHEME_PULL, SCHEME_AA = 0, 1, 2, 3
 STAGES = ["collide", "stream", "noslip", "periodicity", "moments", "velocity_reset",
           "density_reset", "fused"]
 
@@ -355,10 +355,11 @@ class State:
         check(lib().plb_set_tuning(self.h, int(key), int(value)))
 
     def scheme(self):
-        """'push' or 'pull': the streaming scheme of the next fused step."""
+        """'push', 'pull' or 'aa' (one array): the streaming scheme of the
+        next fused step."""
         v = C.c_int()
         check(lib().plb_step_scheme(self.h, C.byref(v)))
-        return "pull" if v.value == SCHEME_PULL else "push"
+        return {SCHEME_PULL: "pull", SCHEME_AA: "aa"}.get(v.value, "push")
 
     def global_moments(self):
         out = np.zeros(4, np.float64)

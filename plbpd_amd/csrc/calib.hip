@@ -156,8 +156,14 @@ int plb_set_tuning(plb_state* s, int key, int value) {
         case PLB_TUNE_NT_LOAD: s->tune_nt_load = value != 0; break;
         case PLB_TUNE_NT_STORE: s->tune_nt_store = value != 0; break;
         case PLB_TUNE_SCHEME:
-            if (value != PLB_SCHEME_AUTO && value != PLB_SCHEME_PUSH && value != PLB_SCHEME_PULL)
+            if (value != PLB_SCHEME_AUTO && value != PLB_SCHEME_PUSH && value != PLB_SCHEME_PULL &&
+                value != PLB_SCHEME_AA)
                 return fail(PLB_ERR_INVALID_ARGUMENT, "unknown streaming scheme");
+            // the scheme follows the number of population arrays the state holds
+            if (s->aa && value != PLB_SCHEME_AA && value != PLB_SCHEME_AUTO)
+                return fail(PLB_ERR_LOGIC, "a one-array state (PLB_MODE_FUSED_AA) steps with PLB_SCHEME_AA only");
+            if (!s->aa && value == PLB_SCHEME_AA)
+                return fail(PLB_ERR_LOGIC, "PLB_SCHEME_AA needs a state created with PLB_MODE_FUSED_AA");
             s->tune_scheme = value;
             break;
         case PLB_TUNE_TILE_ORDER:
@@ -189,7 +195,7 @@ int plb_set_tuning(plb_state* s, int key, int value) {
 int plb_step_scheme(const plb_state* s, int* scheme) {
     if (!s || !scheme) return fail(PLB_ERR_INVALID_ARGUMENT, "null argument");
     if (s->mode != PLB_MODE_FUSED) return fail(PLB_ERR_LOGIC, "literal mode has no fused scheme");
-    *scheme = fused_uses_pull(s) ? PLB_SCHEME_PULL : PLB_SCHEME_PUSH;
+    *scheme = s->aa ? PLB_SCHEME_AA : (fused_uses_pull(s) ? PLB_SCHEME_PULL : PLB_SCHEME_PUSH);
     return PLB_OK;
 }
 

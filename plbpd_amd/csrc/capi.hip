@@ -221,7 +221,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     if (d->q != 9 && d->q != 19 && d->q != 27)
         return fail(PLB_ERR_LOGIC, "Model with q=" + std::to_string(d->q) + " not yet implemented");
     if (d->x <= 0 || d->y <= 0 || d->z <= 0) return fail(PLB_ERR_INVALID_ARGUMENT, "argument is zero");
-    if (d->mode != PLB_MODE_LITERAL && d->mode != PLB_MODE_FUSED)
+    if (d->mode != PLB_MODE_LITERAL && d->mode != PLB_MODE_FUSED && d->mode != PLB_MODE_FUSED_AA)
         return fail(PLB_ERR_INVALID_ARGUMENT, "unknown mode");
     // Cell counts that no device holds are refused before any size product
     // can wrap: the fused kernels number cells, rows and tiles in 32 bits, and
@@ -249,7 +249,11 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->Y = d->y;
     s->Z = d->z;
     s->variant = d->variant;
-    s->mode = d->mode;
+    // a one-array state has fused semantics; only its step differs
+    s->mode = d->mode == PLB_MODE_FUSED_AA ? PLB_MODE_FUSED : d->mode;
+    s->aa = d->mode == PLB_MODE_FUSED_AA;
+    s->aa_odd = false;
+    s->aa_collide = true;
     s->device = d->device;
     s->sbytes = storage_bytes(d->variant);
     s->cur = 0;
@@ -292,6 +296,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
 
     const uint64_t X = d->x, Y = d->y, Z = d->z;
     size_t mom_bytes;
+    if (s->aa) s->tune_scheme = PLB_SCHEME_AA;
     if (d->mode == PLB_MODE_LITERAL) {
         s->ncell = (X + 2) * (Y + 2) * (Z + 2);
         s->pstride = s->ncell;
@@ -312,7 +317,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
         return fail(PLB_ERR_HIP, hipGetErrorString(e));
     }
     const size_t pop_bytes = s->pstride * s->q * s->sbytes;
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < (s->aa ? 1 : 2); ++b) {
         e = hipMalloc(&s->buf[b], pop_bytes);
         if (e != hipSuccess) return cleanup(PLB_ERR_OOM, "population buffer: " + std::string(hipGetErrorString(e)));
         e = hipMemset(s->buf[b], 0, pop_bytes);
@@ -322,7 +327,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     if (e == hipSuccess) e = hipMalloc(&s->u, 3 * mom_bytes);
     if (e == hipSuccess) e = hipMemset(s->rho, 0, mom_bytes);
     if (e == hipSuccess) e = hipMemset(s->u, 0, 3 * mom_bytes);
-    if (e == hipSuccess && d->mode == PLB_MODE_FUSED) {
+    if (e == hipSuccess && s->mode == PLB_MODE_FUSED) {
         e = hipMalloc(&s->flags, s->ncell);
         if (e == hipSuccess) e = hipMemset(s->flags, 0, s->ncell);
     }
@@ -351,7 +356,7 @@ int plb_state_destroy(plb_state* s) {
 
 int plb_state_bytes(const plb_state* s, size_t* bytes) {
     if (!s || !bytes) return fail(PLB_ERR_INVALID_ARGUMENT, "null argument");
-    size_t b = 2 * s->pstride * s->q * s->sbytes;
+    size_t b = (s->aa ? 1 : 2) * s->pstride * s->q * s->sbytes;
     if (s->mode == PLB_MODE_LITERAL)
         b += 4 * s->ncell * s->sbytes;
     else
@@ -490,7 +495,7 @@ int plb_stream(plb_state* s, const char* stream_type) {
     }
     // Fused mode: both schemes give the same numbers; "pull" asks for the
     // pull kernels, which the push scheme knob would contradict.
-    if (t == "pull" && !fused_uses_pull(s))
+    if (t == "pull" && !s->aa && !fused_uses_pull(s))
         return fail(PLB_ERR_LOGIC, "stream type pull with the push scheme selected (PLB_TUNE_SCHEME)");
     const bool collide = s->collide_pending;
     s->collide_pending = false;
@@ -650,8 +655,18 @@ int plb_step_graph(plb_state* s, int nsteps, int steps_per_graph) {
     if (nsteps < 0) return fail(PLB_ERR_INVALID_ARGUMENT, "negative step count");
     // The captured steps keep one storage form: post-collision for the pull
     // step (from the pre-collision form its first step is the collision
-    // alone, run here), pre-collision for the push step.
-    if (fused_uses_pull(s)) {
+    // alone, run here), pre-collision for the push step, and for a one-array
+    // state the form after an even step count (an odd step is run here first
+    // if needed; a replay of an even number of steps returns to the form).
+    if (s->aa) {
+        if (s->aa_odd && nsteps > 0) {
+            rc = timed(s, PLB_STAGE_FUSED, [&] { return fused_launch(s, true); });
+            if (rc) return rc;
+            --nsteps;
+        }
+        rc = fused_to_pre(s);
+        if (rc) return rc;
+    } else if (fused_uses_pull(s)) {
         if (!s->post_collision && nsteps > 0) {
             rc = timed(s, PLB_STAGE_FUSED, [&] { return fused_launch(s, true); });
             if (rc) return rc;

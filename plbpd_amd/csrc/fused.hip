@@ -341,6 +341,132 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
     own_store<Q, V, NTS>(f, B, c, ps, self_solid);
 }
 
+// One-array (AA-pattern) steps.  A state created with PLB_MODE_FUSED_AA keeps
+// one population array and alternates two in-place steps:
+//   even: from form P (population k of cell x in its own slot k, before
+//         collision) each cell reads its own slots, collides, and writes f*_k
+//         into its own slot rev(k): form S;
+//   odd:  from form S each fluid cell gathers f_k = S[rev k][x - c_k] (the
+//         neighbour's f*_k), or S[k][x] (its own f*_{rev k}, halfway
+//         bounce-back) where x - c_k is not fluid; collides; and writes f*_j
+//         to the slot it reads for population rev(j): P[j][x + c_j], or
+//         P[rev j][x] where x + c_j is not fluid -- form P again.
+// Each odd-step cell writes exactly the slots it read and no other cell
+// touches them, so the update needs no second array.  The moving-wall term
+// of Boundary.cpp:213-217 is subtracted when the value that enters a wall is
+// written, with the pull step's expression (wall_terms), so every step gives
+// the pull and push steps' numbers bit for bit.  COLLIDE = false streams
+// without colliding (the stream-only step of the staged calls).
+template <int Q, int V, bool COLLIDE, class I, bool MOV>
+__global__ __launch_bounds__(kBlock) void k_aa_even(typename Prec<V>::S* __restrict__ A,
+                                                    const uint8_t* __restrict__ flags,
+                                                    FusedArgs<typename Prec<V>::T> a) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    const Cell p = cell_of(a);
+    if (!p.valid) return;
+    const I c = (I)p.c;
+    const I ps = (I)a.pstride;
+    T f[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) f[k] = P::load_raw(*elem(A, (I)k * ps + c));
+    const bool self_solid = flags[c] != 0;
+    if (__all(self_solid)) return;
+    const T rho = bgk<Q, P>(f, a, COLLIDE);
+    if (MOV && a.bc.moving && !self_solid) {
+        const Seams<I> sm = seams<I>(p.x, p.y, p.z, a);
+        wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
+    }
+    // solid lanes store zeros, so that every line is written whole
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+        *elem(A, (I)rev_k<Q>(k) * ps + c) = P::store_raw(self_solid ? (T)0 : f[k]);
+}
+
+// The slot of population k that an odd step (or the out-of-place gather)
+// reads for cell c: S[rev k][x - c_k], or S[k][x] where x - c_k is not fluid.
+template <int Q, int K, int NB, class I>
+__device__ inline I aa_slot(I c, I ps, uint32_t m, const Seams<I>& s) {
+    constexpr int R = rev_k<Q>(K);
+    const bool solid = NB ? ((m >> R) & 1u) != 0 : pull_wall<Q, K>(s);
+    const I moved = (I)R * ps + (c + pull_offset<Q, K>(s));
+    const I own = (I)K * ps + c;
+    return moved ^ ((moved ^ own) & ((I)0 - (I)solid));
+}
+
+template <int Q, int K, int NB, class I>
+__device__ inline void aa_slots(I (&slot)[Q], I c, I ps, uint32_t m, const Seams<I>& s) {
+    if constexpr (K < Q) {
+        slot[K] = aa_slot<Q, K, NB, I>(c, ps, m, s);
+        aa_slots<Q, K + 1, NB, I>(slot, c, ps, m, s);
+    }
+}
+
+template <int Q, int V, int NB, bool COLLIDE, class I, bool MOV>
+__global__ __launch_bounds__(kBlock) void k_aa_odd(typename Prec<V>::S* __restrict__ A,
+                                                   const uint8_t* __restrict__ flags,
+                                                   const uint32_t* __restrict__ nbmask,
+                                                   FusedArgs<typename Prec<V>::T> a) {
+    using P = Prec<V>;
+    using T = typename P::T;
+    const Cell p = cell_of(a);
+    if (!p.valid) return;
+    const I c = (I)p.c;
+    uint32_t m = 0;
+    bool self_solid;
+    if (NB) {
+        m = nbmask[c];
+        self_solid = m & 1u;
+    } else {
+        self_solid = flags[c] != 0;
+    }
+    // a solid cell owns no slot of the update: it neither reads nor writes
+    if (self_solid) return;
+    const Seams<I> sm = seams<I>(p.x, p.y, p.z, a);
+    const I ps = (I)a.pstride;
+    I slot[Q];
+    aa_slots<Q, 0, NB, I>(slot, c, ps, m, sm);
+    T f[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) f[k] = P::load_raw(*elem(A, slot[k]));
+    const T rho = bgk<Q, P>(f, a, COLLIDE);
+    if (MOV && a.bc.moving) wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
+    // f*_j goes where population rev(j) was read
+#pragma unroll
+    for (int j = 0; j < Q; ++j) *elem(A, slot[rev_k<Q>(j)]) = P::store_raw(f[j]);
+}
+
+// Form S -> form P out of place (reading a state between an even and an odd
+// step): B[k][x] = the value the odd step would gather; solid cells zero.
+template <int Q, int V, int NB, class I>
+__global__ __launch_bounds__(kBlock) void k_aa_gather(const typename Prec<V>::S* __restrict__ A,
+                                                      typename Prec<V>::S* __restrict__ B,
+                                                      const uint8_t* __restrict__ flags,
+                                                      const uint32_t* __restrict__ nbmask,
+                                                      FusedArgs<typename Prec<V>::T> a) {
+    using P = Prec<V>;
+    const Cell p = cell_of(a);
+    if (!p.valid) return;
+    const I c = (I)p.c;
+    uint32_t m = 0;
+    bool self_solid;
+    if (NB) {
+        m = nbmask[c];
+        self_solid = m & 1u;
+    } else {
+        self_solid = flags[c] != 0;
+    }
+    const Seams<I> sm = seams<I>(p.x, p.y, p.z, a);
+    const I ps = (I)a.pstride;
+    I slot[Q];
+    aa_slots<Q, 0, NB, I>(slot, c, ps, m, sm);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const typename P::S v = *elem(A, slot[k]);
+        *elem(B, (I)k * ps + c) = self_solid ? P::store_raw((typename P::T)0) : v;
+    }
+}
+
 // Collision only, in place (pre-collision -> post-collision storage).
 template <int Q, int V>
 __global__ __launch_bounds__(kBlock) void k_collide_only(typename Prec<V>::S* __restrict__ A,
@@ -675,6 +801,7 @@ int fused_initialize(plb_state* s, double rho0, const double u0[3], const plb_in
         constexpr int V = decltype(vc)::value;
         using S = typename Prec<V>::S;
         s->post_collision = false;
+        s->aa_odd = false;
         hipLaunchKernelGGL((k_finit<Q, V>), dim3(blocks_for(s->ncell, kBlock)), dim3(kBlock), 0,
                            s->stream, (S*)s->buf[s->cur], (uint32_t)s->X, (uint32_t)s->Y,
                            (uint32_t)s->Z, (uint64_t)s->pstride, rho0, u0[0], u0[1], u0[2], kind, seed,
@@ -686,7 +813,7 @@ int fused_initialize(plb_state* s, double rho0, const double u0[3], const plb_in
 
 namespace {
 
-enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly };
+enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly, kAaEven, kAaOdd, kAaGather };
 
 // Launch geometry and arguments shared by every fused-layout kernel.
 int launch_kind(plb_state* s, KernelKind kind) {
@@ -833,11 +960,49 @@ int launch_kind(plb_state* s, KernelKind kind) {
                 hipLaunchKernelGGL((k_collide_only<Q, V>), grid, block, 0, s->stream, (S*)s->buf[s->cur],
                                    (const uint8_t*)s->flags, a);
                 break;
+            case kAaEven:
+            case kAaOdd:
+            case kAaGather: {
+                S* A1 = (S*)s->buf[s->cur];
+                auto go = [&](auto ic, auto nbc, auto colc, auto mvc) {
+                    using I = decltype(ic);
+                    constexpr int NBv = decltype(nbc)::value;
+                    constexpr bool Cv = decltype(colc)::value, MVv = decltype(mvc)::value;
+                    if (kind == kAaEven)
+                        hipLaunchKernelGGL((k_aa_even<Q, V, Cv, I, MVv>), grid, block, 0, s->stream, A1,
+                                           (const uint8_t*)s->flags, a);
+                    else if (kind == kAaOdd)
+                        hipLaunchKernelGGL((k_aa_odd<Q, V, NBv, Cv, I, MVv>), grid, block, 0, s->stream, A1,
+                                           (const uint8_t*)s->flags, s->nbmask, a);
+                    else
+                        hipLaunchKernelGGL((k_aa_gather<Q, V, NBv, I>), grid, block, 0, s->stream, A1, B,
+                                           (const uint8_t*)s->flags, s->nbmask, a);
+                };
+                using N0 = std::integral_constant<int, 0>;
+                using N1 = std::integral_constant<int, 1>;
+                using F = std::false_type;
+                using Tr = std::true_type;
+                const bool mv = a.bc.moving != 0, col = s->aa_collide;
+                auto by_nb = [&](auto ic) {
+                    auto by_col = [&](auto nbc) {
+                        if (kind == kAaGather) return go(ic, nbc, F{}, F{});
+                        if (col) mv ? go(ic, nbc, Tr{}, Tr{}) : go(ic, nbc, Tr{}, F{});
+                        else mv ? go(ic, nbc, F{}, Tr{}) : go(ic, nbc, F{}, F{});
+                    };
+                    // the even step reads no neighbour, so no mask instantiation
+                    if (nb && kind != kAaEven) by_col(N1{});
+                    else by_col(N0{});
+                };
+                if (idx32) by_nb(uint32_t());
+                else by_nb(uint64_t());
+                break;
+            }
         }
         PLB_HIP_TRY(hipGetLastError());
         return PLB_OK;
     });
-    if (rc == PLB_OK && kind != kCollideOnly) s->cur ^= 1;
+    // every kernel but the in-place ones writes the other buffer
+    if (rc == PLB_OK && kind != kCollideOnly && kind != kAaEven && kind != kAaOdd) s->cur ^= 1;
     return rc;
 }
 
@@ -847,8 +1012,37 @@ int launch_kind(plb_state* s, KernelKind kind) {
 bool fused_uses_pull(const plb_state* s) {
     // the pull step is the faster one for every storage type and boundary on
     // MI355X (whole-line stores; moving walls by write-time terms)
-    return s->tune_scheme != PLB_SCHEME_PUSH;
+    return !s->aa && s->tune_scheme != PLB_SCHEME_PUSH;
 }
+
+namespace {
+// A one-array state between an even and an odd step (form S) is read after
+// its populations are gathered into a second array, allocated for the
+// purpose; the old array is then freed, so the state holds one array again.
+int aa_to_pre(plb_state* s) {
+    if (!s->aa_odd) return PLB_OK;
+    const size_t pop_bytes = s->pstride * s->q * s->sbytes;
+    void* tmp = nullptr;
+    PLB_HIP_TRY(hipMalloc(&tmp, pop_bytes));
+    s->buf[s->cur ^ 1] = tmp;
+    int rc = launch_kind(s, kAaGather);  // writes buf[cur ^ 1] and swaps
+    if (rc == PLB_OK) {
+        hipError_t e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess) e = hipFree(s->buf[s->cur ^ 1]);
+        if (e != hipSuccess) rc = fail(PLB_ERR_HIP, hipGetErrorString(e));
+        s->buf[s->cur ^ 1] = nullptr;
+        s->aa_odd = false;
+        // a captured step graph holds the old array's address
+        if (s->graph) (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+        s->graph_steps = 0;
+    } else {
+        (void)hipFree(tmp);
+        s->buf[s->cur ^ 1] = nullptr;
+    }
+    return rc;
+}
+}  // namespace
 
 // The populations in buf[cur] are stored either before collision (what the
 // reference's n holds after a whole step) or, between pull steps, after the
@@ -857,6 +1051,7 @@ bool fused_uses_pull(const plb_state* s) {
 // later step gathers (finishing the pending streaming), then collides.
 // Reading the state performs the pending gather first.
 int fused_to_pre(plb_state* s) {
+    if (s->aa) return aa_to_pre(s);
     if (!s->post_collision) return PLB_OK;
     const int rc = launch_kind(s, kGather);
     if (rc == PLB_OK) s->post_collision = false;
@@ -865,6 +1060,13 @@ int fused_to_pre(plb_state* s) {
 
 int fused_launch(plb_state* s, bool collide) {
     int rc;
+    if (s->aa) {
+        s->aa_collide = collide;
+        rc = launch_kind(s, s->aa_odd ? kAaOdd : kAaEven);
+        if (rc == PLB_OK) s->aa_odd = !s->aa_odd;
+        s->moments_valid = false;
+        return rc;
+    }
     if (!collide) {
         rc = fused_to_pre(s);
         if (rc == PLB_OK) rc = launch_kind(s, kStreamOnly);
@@ -973,6 +1175,7 @@ int fused_transfer(plb_state* s, int field, int layout, int dtype, void* host, b
     if (rc == PLB_OK && upload) {
         s->moments_valid = false;
         s->post_collision = false;  // uploaded populations are pre-collision
+        s->aa_odd = false;
     }
     return rc;
 }

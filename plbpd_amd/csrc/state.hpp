@@ -103,6 +103,10 @@ struct plb_state {
     int tune_vector;      // row pull kernel cells per lane (1, 2, 4) where X allows; 0 off; -1 auto
     int tune_lit_fuse;    // literal: 1 collision and push in one kernel when they follow each other
     bool post_collision;  // fused: buf[cur] holds post-collision populations (pull step)
+    // one-array state (PLB_MODE_FUSED_AA): buf[cur] only; aa_odd = after an
+    // even step, f*_k swapped into slot rev(k); aa_collide = the step in
+    // progress collides
+    bool aa, aa_odd, aa_collide;
     // timers
     bool timers_on;
     std::vector<plb::TimerPair> pending;
