@@ -280,6 +280,8 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
 #define PLB_SCHEME_PUSH 1   /*   collide, then scatter to neighbours            */
 #define PLB_SCHEME_PULL 2   /*   gather from neighbours, then collide           */
 #define PLB_SCHEME_AA 3     /*   one array (PLB_MODE_FUSED_AA states only)      */
+#define PLB_SCHEME_PULL_TWO 4 /* reported only: pull, two steps per launch       */
+                              /* (chosen by PLB_TUNE_TWO_STEP)                   */
 #define PLB_TUNE_TILE_ORDER 4 /* rows in B x B blocks of the y-z plane (B a power */
                               /* of two dividing the row counts; default 0 =      */
                               /* linear)                                          */
@@ -308,9 +310,19 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
                                 /* kernel (same rho and u), inside plb_step and for   */
                                 /* the staged calls alike; 0 runs one kernel per      */
                                 /* reference call                                      */
+#define PLB_TUNE_TWO_STEP 9     /* fused pull step: 1 advances two steps per launch   */
+                                /* (one read and one write of the populations per two */
+                                /* steps, the same values bit for bit) wherever a     */
+                                /* kernel exists: D3Q19, and D3Q27 with 2- or 4-byte  */
+                                /* storage, x a multiple of 64, 32-bit offsets; 0     */
+                                /* one step per launch; -1 (default) automatic: on    */
+                                /* for 4-byte storage                                  */
+#define PLB_TUNE_TWO_ZCHUNK 10  /* planes per z-chunk of a two-step launch (0: chosen */
+                                /* from the grid and the compute-unit count)          */
 int plb_set_tuning(plb_state* s, int key, int value);
-/* The streaming scheme (PLB_SCHEME_PUSH or PLB_SCHEME_PULL) the next fused
- * step will use. */
+/* The streaming scheme the next fused steps will use: PLB_SCHEME_PUSH,
+ * PLB_SCHEME_PULL, PLB_SCHEME_PULL_TWO (pull, two steps per launch, for
+ * pairs of steps) or PLB_SCHEME_AA. */
 int plb_step_scheme(const plb_state* s, int* scheme);
 
 /* ---- batched domains (no reference counterpart: one domain per PLBPD,

@@ -30,8 +30,8 @@ LAYOUT_REFERENCE, LAYOUT_COMPACT = 0, 1
 DTYPE_F64, DTYPE_F32 = 0, 1
 INIT_UNIFORM, INIT_PERTURBED, INIT_TAYLOR_GREEN = 0, 1, 2
 (TUNE_BLOCK_X, TUNE_NT_LOAD, TUNE_NT_STORE, TUNE_SCHEME, TUNE_TILE_ORDER, TUNE_VECTOR, TUNE_XCD, TUNE_INDEX64,
- TUNE_LITERAL_FUSE) = range(9)
-SCHEME_AUTO, SCHEME_PUSH, SCHEME_PULL, SCHEME_AA = 0, 1, 2, 3
+ TUNE_LITERAL_FUSE, TUNE_TWO_STEP, TUNE_TWO_ZCHUNK) = range(11)
+SCHEME_AUTO, SCHEME_PUSH, SCHEME_PULL, SCHEME_AA, SCHEME_PULL_TWO = 0, 1, 2, 3, 4
 STAGES = ["collide", "stream", "noslip", "periodicity", "moments", "velocity_reset",
           "density_reset", "fused"]
 
@@ -355,11 +355,15 @@ class State:
         check(lib().plb_set_tuning(self.h, int(key), int(value)))
 
     def scheme(self):
-        """'push', 'pull' or 'aa' (one array): the streaming scheme of the
-        next fused step."""
+        """'push', 'pull', 'pull2' (pull, two steps per launch) or 'aa' (one
+        array): the streaming scheme of the next fused steps."""
         v = C.c_int()
         check(lib().plb_step_scheme(self.h, C.byref(v)))
-        return {SCHEME_PULL: "pull", SCHEME_AA: "aa"}.get(v.value, "push")
+        return {SCHEME_PULL: "pull", SCHEME_AA: "aa", SCHEME_PULL_TWO: "pull2"}.get(v.value, "push")
+
+    def steps_per_launch(self):
+        """Steps one launch of the fused kernel advances (2 for 'pull2')."""
+        return 2 if self.scheme() == "pull2" else 1
 
     def global_moments(self):
         out = np.zeros(4, np.float64)

@@ -177,6 +177,14 @@ int plb_set_tuning(plb_state* s, int key, int value) {
             break;
         case PLB_TUNE_INDEX64: s->tune_index64 = value != 0; break;
         case PLB_TUNE_LITERAL_FUSE: s->tune_lit_fuse = value != 0; break;
+        case PLB_TUNE_TWO_STEP:
+            if (value < -1 || value > 1) return fail(PLB_ERR_INVALID_ARGUMENT, "two-step setting must be -1, 0 or 1");
+            s->tune_two = value;
+            break;
+        case PLB_TUNE_TWO_ZCHUNK:
+            if (value < 0) return fail(PLB_ERR_INVALID_ARGUMENT, "z-chunk length must be 0 (automatic) or positive");
+            s->tune_two_zc = value;
+            break;
         case PLB_TUNE_VECTOR:
             if (value != -1 && value != 0 && value != 1 && value != 2 && value != 4)
                 return fail(PLB_ERR_INVALID_ARGUMENT, "row pull width must be -1, 0, 1, 2 or 4 cells per lane");
@@ -195,7 +203,12 @@ int plb_set_tuning(plb_state* s, int key, int value) {
 int plb_step_scheme(const plb_state* s, int* scheme) {
     if (!s || !scheme) return fail(PLB_ERR_INVALID_ARGUMENT, "null argument");
     if (s->mode != PLB_MODE_FUSED) return fail(PLB_ERR_LOGIC, "literal mode has no fused scheme");
-    *scheme = s->aa ? PLB_SCHEME_AA : (fused_uses_pull(s) ? PLB_SCHEME_PULL : PLB_SCHEME_PUSH);
+    if (s->aa)
+        *scheme = PLB_SCHEME_AA;
+    else if (!fused_uses_pull(s))
+        *scheme = PLB_SCHEME_PUSH;
+    else
+        *scheme = fused_two_eligible(s) ? PLB_SCHEME_PULL_TWO : PLB_SCHEME_PULL;
     return PLB_OK;
 }
 

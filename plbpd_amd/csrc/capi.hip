@@ -291,6 +291,14 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->tune_xcd = -1;
     s->tune_index64 = 0;
     s->tune_lit_fuse = 1;
+    s->tune_two = -1;
+    s->tune_two_zc = 0;
+    s->num_cu = 256;
+    {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, d->device) == hipSuccess && cu > 0)
+            s->num_cu = cu;
+    }
     s->post_collision = false;
     for (double& v : s->stage_ms) v = 0.0;
 
@@ -637,6 +645,10 @@ int plb_step(plb_state* s, int nsteps) {
             if ((rc = timed(s, PLB_STAGE_MOMENTS, [&] { return lit_moments(s); }))) return rc;
             if ((rc = timed(s, PLB_STAGE_RESET_VELOCITY, [&] { return lit_reset_velocity(s, bc); }))) return rc;
             if ((rc = timed(s, PLB_STAGE_RESET_DENSITY, [&] { return lit_reset_density(s, bc); }))) return rc;
+        } else if (i + 1 < nsteps && s->post_collision && fused_two_eligible(s)) {
+            // two steps in one pass over the populations
+            if ((rc = timed(s, PLB_STAGE_FUSED, [&] { return fused_launch_two(s); }))) return rc;
+            ++i;
         } else {
             if ((rc = timed(s, PLB_STAGE_FUSED, [&] { return fused_launch(s, true); }))) return rc;
         }
@@ -682,8 +694,17 @@ int plb_step_graph(plb_state* s, int nsteps, int steps_per_graph) {
         const bool timers = s->timers_on;
         s->timers_on = false;
         PLB_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        // two-step launches where eligible, in a number that keeps the count
+        // of buffer swaps even (the graph must leave buf[cur] where it began)
+        int pairs = !s->aa && s->post_collision && fused_two_eligible(s) ? steps_per_graph / 2 : 0;
+        int singles = steps_per_graph - 2 * pairs;
+        if (((pairs + singles) & 1) && pairs > 0) {
+            --pairs;
+            singles += 2;
+        }
         int lrc = PLB_OK;
-        for (int i = 0; i < steps_per_graph && lrc == PLB_OK; ++i) lrc = fused_launch(s, true);
+        for (int i = 0; i < pairs && lrc == PLB_OK; ++i) lrc = fused_launch_two(s);
+        for (int i = 0; i < singles && lrc == PLB_OK; ++i) lrc = fused_launch(s, true);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s->stream, &g);
         s->timers_on = timers;

@@ -16,6 +16,7 @@
 // arithmetic; a neighbour that is a wall or solid sends the population back
 // to the cell's own reverse slot (halfway bounce-back, with the moving-wall
 // term of Boundary.cpp:213-217).
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -125,159 +126,6 @@ __global__ __launch_bounds__(kBlock) void k_push(const typename Prec<V>::S* __re
             __builtin_nontemporal_store(P::store_raw(v), B + dst);
         else
             B[dst] = P::store_raw(v);
-    }
-}
-
-// Element offsets from a cell to its x-1, x+1, y-1, y+1, z-1, z+1
-// neighbours with the periodic wrap folded in, in the unsigned index type I
-// (modular arithmetic: a "negative" offset wraps and the sum lands in range),
-// and whether each of those neighbour layers is a wall layer (a walled axis
-// never wraps for a fluid cell: the wall layer sits between it and the edge).
-template <class I>
-struct Seams {
-    I xm, xp, ym, yp, zm, zp;
-    bool wxm, wxp, wym, wyp, wzm, wzp;
-};
-
-template <class I, class T>
-__device__ inline Seams<I> seams(uint32_t x, uint32_t y, uint32_t z, const FusedArgs<T>& a) {
-    const I X = a.X, XY = (I)a.X * a.Y;
-    Seams<I> s;
-    s.xm = x == 0 ? (I)(a.X - 1) : (I)0 - (I)1;
-    s.xp = x + 1 == a.X ? (I)0 - (I)(a.X - 1) : (I)1;
-    s.ym = y == 0 ? (I)(a.Y - 1) * X : (I)0 - X;
-    s.yp = y + 1 == a.Y ? (I)0 - (I)(a.Y - 1) * X : X;
-    s.zm = z == 0 ? (I)(a.Z - 1) * XY : (I)0 - XY;
-    s.zp = z + 1 == a.Z ? (I)0 - (I)(a.Z - 1) * XY : XY;
-    s.wxm = a.bc.wall_lo[0] && x == 1;
-    s.wxp = a.bc.wall_hi[0] && x + 2 == a.X;
-    s.wym = a.bc.wall_lo[1] && y == 1;
-    s.wyp = a.bc.wall_hi[1] && y + 2 == a.Y;
-    s.wzm = a.bc.wall_lo[2] && z == 1;
-    s.wzp = a.bc.wall_hi[2] && z + 2 == a.Z;
-    return s;
-}
-
-// Offset from a cell to x - c_k (the pull source), and whether that source is
-// a wall layer.  The direction components are compile-time constants, so each
-// is at most three adds and three ors.
-template <int Q, int K, class I>
-__device__ inline I pull_offset(const Seams<I>& s) {
-    constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
-    I d = 0;
-    if (cx > 0) d += s.xm;
-    if (cx < 0) d += s.xp;
-    if (cy > 0) d += s.ym;
-    if (cy < 0) d += s.yp;
-    if (cz > 0) d += s.zm;
-    if (cz < 0) d += s.zp;
-    return d;
-}
-template <int Q, int K, class I>
-__device__ inline bool pull_wall(const Seams<I>& s) {
-    constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
-    bool w = false;
-    if (cx > 0) w = w || s.wxm;
-    if (cx < 0) w = w || s.wxp;
-    if (cy > 0) w = w || s.wym;
-    if (cy < 0) w = w || s.wyp;
-    if (cz > 0) w = w || s.wzm;
-    if (cz < 0) w = w || s.wzp;
-    return w;
-}
-
-// Element i of a population buffer.  With 32-bit indices the launcher
-// guarantees that every byte offset fits 32 bits, and the access is written
-// as buffer base + 32-bit byte offset, which the compiler turns into a load
-// or store with a scalar base address (no 64-bit address add per access).
-template <class S, class I>
-__device__ inline const S* elem(const S* base, I i) {
-    if constexpr (sizeof(I) == 4)
-        return reinterpret_cast<const S*>(reinterpret_cast<const char*>(base) + i * (I)sizeof(S));
-    else
-        return base + i;
-}
-template <class S, class I>
-__device__ inline S* elem(S* base, I i) {
-    if constexpr (sizeof(I) == 4)
-        return reinterpret_cast<S*>(reinterpret_cast<char*>(base) + i * (I)sizeof(S));
-    else
-        return base + i;
-}
-
-template <int Q, int K, int V, int NB, bool NT, class I>
-__device__ inline void pull_gather(typename Prec<V>::T (&f)[Q], const typename Prec<V>::S* __restrict__ A,
-                                   I c, I ps, uint32_t m, const Seams<I>& s) {
-    if constexpr (K < Q) {
-        using P = Prec<V>;
-        constexpr int R = rev_k<Q>(K);
-        // source x - c_k; it is solid when neighbour rev(k) is
-        const bool solid = NB ? ((m >> R) & 1u) != 0 : pull_wall<Q, K>(s);
-        const I moved = (I)K * ps + (c + pull_offset<Q, K>(s));
-        const I own = (I)R * ps + c;
-        // branch-free select (a conditional here becomes divergent branches)
-        const I src = moved ^ ((moved ^ own) & ((I)0 - (I)solid));
-        const typename P::S* p = elem(A, src);
-        f[K] = NT ? P::load_raw(__builtin_nontemporal_load(p)) : P::load_raw(*p);
-        pull_gather<Q, K + 1, V, NB, NT>(f, A, c, ps, m, s);
-    }
-}
-
-// Around interior solids with 2-byte storage: both candidates of every
-// population, the pulled value and the cell's own reverse slot, are loaded
-// before the neighbour mask is known (no load address waits on it); the mask
-// then selects.  On MI355X this made the 2-byte step 11% (D3Q19) to 13%
-// (D3Q27) faster; 4-byte storage was unchanged and 8-byte 2-3% slower, so
-// they keep the single load at a mask-selected address.
-template <int Q, int K, int V, class I>
-__device__ inline void pull_gather_both(typename Prec<V>::S (&fv)[Q], typename Prec<V>::S (&own)[Q],
-                                        const typename Prec<V>::S* __restrict__ A, I c, I ps,
-                                        const Seams<I>& s) {
-    if constexpr (K < Q) {
-        fv[K] = *elem(A, (I)K * ps + (c + pull_offset<Q, K>(s)));
-        if constexpr (K > 0) own[K] = *elem(A, (I)rev_k<Q>(K) * ps + c);
-        pull_gather_both<Q, K + 1, V>(fv, own, A, c, ps, s);
-    }
-}
-
-template <int Q, int V, bool NT, class I>
-__device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Prec<V>::S* __restrict__ B, I c,
-                                 I ps, bool zero) {
-    using P = Prec<V>;
-    using T = typename P::T;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const typename P::S v = P::store_raw(zero ? (T)0 : f[k]);
-        typename P::S* p = elem(B, (I)k * ps + c);
-        if (NT)
-            __builtin_nontemporal_store(v, p);
-        else
-            *p = v;
-    }
-}
-
-// Moving walls in the pull scheme.  A population f*_k that a fluid cell sends
-// towards a wall comes back to the same cell as direction rev(k) at the next
-// step (halfway bounce-back), read from the cell's own slot k, and nothing
-// else reads that slot.  So the moving-wall term of Boundary.cpp:213-217 is
-// subtracted when the slot is written, with the rho and the expression of the
-// push step (which subtracts it on the way): the two schemes store the same
-// values.  Only wall faces move; interior solids take no term.
-template <int Q, int K, class P, class I>
-__device__ inline void wall_terms(typename P::T (&f)[Q], typename P::T rho, const uint8_t* __restrict__ flags, I c,
-                                  const Seams<I>& s, const FusedBC<typename P::T>& bc) {
-    using T = typename P::T;
-    if constexpr (K < Q) {
-        constexpr int R = rev_k<Q>(K);
-        constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
-        // neighbour x + c_K (the pull source of direction rev(K)) on a wall face
-        if (pull_wall<Q, R>(s)) {
-            const int fl = flags[c + pull_offset<Q, R>(s)];
-            const T cub = (T)cx * bc.vel[fl][0] + (T)cy * bc.vel[fl][1] + (T)cz * bc.vel[fl][2];
-            const T term = (T)2 * w_scaled<Q, P>(K) * rho * cub * (T)3;
-            f[K] = f[K] - term;
-        }
-        wall_terms<Q, K + 1, P>(f, rho, flags, c, s, bc);
     }
 }
 
@@ -813,7 +661,34 @@ int fused_initialize(plb_state* s, double rho0, const double u0[3], const plb_in
 
 namespace {
 
-enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly, kAaEven, kAaOdd, kAaGather };
+enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly, kAaEven, kAaOdd, kAaGather, kPull2 };
+
+// Chunks of a two-step launch along z.  A workgroup holds a compute unit's
+// LDS, so the launch runs in rounds of num_cu workgroups, and each chunk of
+// zc planes costs zc + 2 planes of work (the grown planes at its ends): the
+// chunk count minimises rounds x (zc + 2).
+TwoGeom two_geom(const plb_state* s) {
+    const int by = two_rows(s->q, (int)s->sbytes);
+    TwoGeom g;
+    g.ntx = (uint32_t)s->X / 64u;
+    g.nty = ((uint32_t)s->Y + (uint32_t)by - 1u) / (uint32_t)by;
+    const uint64_t tiles = (uint64_t)g.ntx * g.nty, Z = (uint64_t)s->Z, ncu = (uint64_t)s->num_cu;
+    uint64_t zc = Z;
+    if (s->tune_two_zc > 0) {
+        zc = std::min<uint64_t>((uint64_t)s->tune_two_zc, Z);
+    } else {
+        uint64_t best = ~0ull;
+        for (uint64_t n = 1; n <= 32 && n <= Z; ++n) {
+            const uint64_t c = (Z + n - 1) / n, chunks = (Z + c - 1) / c;
+            const uint64_t rounds = (tiles * chunks + ncu - 1) / ncu, cost = rounds * (c + 2);
+            if (cost < best) best = cost, zc = c;
+        }
+    }
+    g.zc = (uint32_t)zc;
+    g.nzc = (uint32_t)((Z + zc - 1) / zc);
+    g.xcd = g.nty % 8u == 0u ? 1u : 0u;
+    return g;
+}
 
 // Launch geometry and arguments shared by every fused-layout kernel.
 int launch_kind(plb_state* s, KernelKind kind) {
@@ -917,6 +792,15 @@ int launch_kind(plb_state* s, KernelKind kind) {
                 if (nb) hipLaunchKernelGGL((k_push<Q, V, 1, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
                 else hipLaunchKernelGGL((k_push<Q, V, 0, false>), grid, block, 0, s->stream, A, B, s->flags, s->nbmask, a);
                 break;
+            case kPull2:
+                if constexpr (Q == 9) {
+                    return fail(PLB_ERR_LOGIC, "no two-step kernel for D2Q9");
+                } else {
+                    if (!launch_pull2<Q, V>(a, two_geom(s), nb, a.bc.moving != 0, a.nt_store != 0, A, B, s->flags,
+                                            s->nbmask, s->stream))
+                        return fail(PLB_ERR_LOGIC, "no two-step kernel for this storage");
+                }
+                break;
             case kPull:
                 if (vec) {
                     if (!launch_pull_row<Q, V>(a, vec, a.nt_store != 0, nb, A, B, s->flags, s->nbmask, grid.x,
@@ -1008,6 +892,27 @@ int launch_kind(plb_state* s, KernelKind kind) {
 
 
 }  // namespace
+
+bool fused_two_eligible(const plb_state* s) {
+    if (s->tune_two == 0 || s->aa || !fused_uses_pull(s) || s->mode != PLB_MODE_FUSED) return false;
+    // automatic: 4-byte storage, where two steps per pass measured 1.25x
+    // (D3Q19 256^3 FP32-shifted, 10% solids) to 1.5x (walls only) faster on
+    // MI355X; 2-byte storage ran slower (its step is closer to the
+    // arithmetic limit) and 8-byte storage slower with solids (3-row tiles)
+    if (s->tune_two < 0 && s->sbytes != 4) return false;
+    if ((s->q != 19 && s->q != 27) || two_rows(s->q, (int)s->sbytes) == 0) return false;
+    if (s->X % 64 != 0 || s->tune_index64) return false;
+    // the kernel's offsets are 32-bit
+    return (uint64_t)s->q * s->pstride * s->sbytes <= 0xffffffffull;
+}
+
+int fused_launch_two(plb_state* s) {
+    if (!s->post_collision || !fused_two_eligible(s))
+        return fail(PLB_ERR_LOGIC, "two-step launch needs an eligible state in post-collision form");
+    const int rc = launch_kind(s, kPull2);
+    s->moments_valid = false;
+    return rc;
+}
 
 bool fused_uses_pull(const plb_state* s) {
     // the pull step is the faster one for every storage type and boundary on

@@ -1,6 +1,6 @@
-// Device-side pieces shared by the fused-layout kernels (fused.hip and
-// fused_row.hpp): launch arguments, the block -> cell mapping and the BGK
-// update.  Layout in HBM (structure of arrays, no halo):
+// Device-side pieces shared by the fused-layout kernels (fused.hip,
+// fused_row.hpp and fused_two.hpp): launch arguments, the block -> cell
+// mapping, neighbour offsets, gathers and stores, and the BGK update.  Layout in HBM (structure of arrays, no halo):
 //   f[k * pstride + c],  c = (z*Y + y)*X + x,  x fastest.
 #pragma once
 
@@ -97,6 +97,159 @@ __device__ inline Cell cell_of(const FusedArgs<T>& a) {
     return p;
 }
 
+// Element offsets from a cell to its x-1, x+1, y-1, y+1, z-1, z+1
+// neighbours with the periodic wrap folded in, in the unsigned index type I
+// (modular arithmetic: a "negative" offset wraps and the sum lands in range),
+// and whether each of those neighbour layers is a wall layer (a walled axis
+// never wraps for a fluid cell: the wall layer sits between it and the edge).
+template <class I>
+struct Seams {
+    I xm, xp, ym, yp, zm, zp;
+    bool wxm, wxp, wym, wyp, wzm, wzp;
+};
+
+template <class I, class T>
+__device__ inline Seams<I> seams(uint32_t x, uint32_t y, uint32_t z, const FusedArgs<T>& a) {
+    const I X = a.X, XY = (I)a.X * a.Y;
+    Seams<I> s;
+    s.xm = x == 0 ? (I)(a.X - 1) : (I)0 - (I)1;
+    s.xp = x + 1 == a.X ? (I)0 - (I)(a.X - 1) : (I)1;
+    s.ym = y == 0 ? (I)(a.Y - 1) * X : (I)0 - X;
+    s.yp = y + 1 == a.Y ? (I)0 - (I)(a.Y - 1) * X : X;
+    s.zm = z == 0 ? (I)(a.Z - 1) * XY : (I)0 - XY;
+    s.zp = z + 1 == a.Z ? (I)0 - (I)(a.Z - 1) * XY : XY;
+    s.wxm = a.bc.wall_lo[0] && x == 1;
+    s.wxp = a.bc.wall_hi[0] && x + 2 == a.X;
+    s.wym = a.bc.wall_lo[1] && y == 1;
+    s.wyp = a.bc.wall_hi[1] && y + 2 == a.Y;
+    s.wzm = a.bc.wall_lo[2] && z == 1;
+    s.wzp = a.bc.wall_hi[2] && z + 2 == a.Z;
+    return s;
+}
+
+// Offset from a cell to x - c_k (the pull source), and whether that source is
+// a wall layer.  The direction components are compile-time constants, so each
+// is at most three adds and three ors.
+template <int Q, int K, class I>
+__device__ inline I pull_offset(const Seams<I>& s) {
+    constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
+    I d = 0;
+    if (cx > 0) d += s.xm;
+    if (cx < 0) d += s.xp;
+    if (cy > 0) d += s.ym;
+    if (cy < 0) d += s.yp;
+    if (cz > 0) d += s.zm;
+    if (cz < 0) d += s.zp;
+    return d;
+}
+template <int Q, int K, class I>
+__device__ inline bool pull_wall(const Seams<I>& s) {
+    constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
+    bool w = false;
+    if (cx > 0) w = w || s.wxm;
+    if (cx < 0) w = w || s.wxp;
+    if (cy > 0) w = w || s.wym;
+    if (cy < 0) w = w || s.wyp;
+    if (cz > 0) w = w || s.wzm;
+    if (cz < 0) w = w || s.wzp;
+    return w;
+}
+
+// Element i of a population buffer.  With 32-bit indices the launcher
+// guarantees that every byte offset fits 32 bits, and the access is written
+// as buffer base + 32-bit byte offset, which the compiler turns into a load
+// or store with a scalar base address (no 64-bit address add per access).
+template <class S, class I>
+__device__ inline const S* elem(const S* base, I i) {
+    if constexpr (sizeof(I) == 4)
+        return reinterpret_cast<const S*>(reinterpret_cast<const char*>(base) + i * (I)sizeof(S));
+    else
+        return base + i;
+}
+template <class S, class I>
+__device__ inline S* elem(S* base, I i) {
+    if constexpr (sizeof(I) == 4)
+        return reinterpret_cast<S*>(reinterpret_cast<char*>(base) + i * (I)sizeof(S));
+    else
+        return base + i;
+}
+
+template <int Q, int K, int V, int NB, bool NT, class I>
+__device__ inline void pull_gather(typename Prec<V>::T (&f)[Q], const typename Prec<V>::S* __restrict__ A,
+                                   I c, I ps, uint32_t m, const Seams<I>& s) {
+    if constexpr (K < Q) {
+        using P = Prec<V>;
+        constexpr int R = rev_k<Q>(K);
+        // source x - c_k; it is solid when neighbour rev(k) is
+        const bool solid = NB ? ((m >> R) & 1u) != 0 : pull_wall<Q, K>(s);
+        const I moved = (I)K * ps + (c + pull_offset<Q, K>(s));
+        const I own = (I)R * ps + c;
+        // branch-free select (a conditional here becomes divergent branches)
+        const I src = moved ^ ((moved ^ own) & ((I)0 - (I)solid));
+        const typename P::S* p = elem(A, src);
+        f[K] = NT ? P::load_raw(__builtin_nontemporal_load(p)) : P::load_raw(*p);
+        pull_gather<Q, K + 1, V, NB, NT>(f, A, c, ps, m, s);
+    }
+}
+
+// Around interior solids with 2-byte storage: both candidates of every
+// population, the pulled value and the cell's own reverse slot, are loaded
+// before the neighbour mask is known (no load address waits on it); the mask
+// then selects.  On MI355X this made the 2-byte step 11% (D3Q19) to 13%
+// (D3Q27) faster; 4-byte storage was unchanged and 8-byte 2-3% slower, so
+// they keep the single load at a mask-selected address.
+template <int Q, int K, int V, class I>
+__device__ inline void pull_gather_both(typename Prec<V>::S (&fv)[Q], typename Prec<V>::S (&own)[Q],
+                                        const typename Prec<V>::S* __restrict__ A, I c, I ps,
+                                        const Seams<I>& s) {
+    if constexpr (K < Q) {
+        fv[K] = *elem(A, (I)K * ps + (c + pull_offset<Q, K>(s)));
+        if constexpr (K > 0) own[K] = *elem(A, (I)rev_k<Q>(K) * ps + c);
+        pull_gather_both<Q, K + 1, V>(fv, own, A, c, ps, s);
+    }
+}
+
+template <int Q, int V, bool NT, class I>
+__device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Prec<V>::S* __restrict__ B, I c,
+                                 I ps, bool zero) {
+    using P = Prec<V>;
+    using T = typename P::T;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const typename P::S v = P::store_raw(zero ? (T)0 : f[k]);
+        typename P::S* p = elem(B, (I)k * ps + c);
+        if (NT)
+            __builtin_nontemporal_store(v, p);
+        else
+            *p = v;
+    }
+}
+
+// Moving walls in the pull scheme.  A population f*_k that a fluid cell sends
+// towards a wall comes back to the same cell as direction rev(k) at the next
+// step (halfway bounce-back), read from the cell's own slot k, and nothing
+// else reads that slot.  So the moving-wall term of Boundary.cpp:213-217 is
+// subtracted when the slot is written, with the rho and the expression of the
+// push step (which subtracts it on the way): the two schemes store the same
+// values.  Only wall faces move; interior solids take no term.
+template <int Q, int K, class P, class I>
+__device__ inline void wall_terms(typename P::T (&f)[Q], typename P::T rho, const uint8_t* __restrict__ flags, I c,
+                                  const Seams<I>& s, const FusedBC<typename P::T>& bc) {
+    using T = typename P::T;
+    if constexpr (K < Q) {
+        constexpr int R = rev_k<Q>(K);
+        constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
+        // neighbour x + c_K (the pull source of direction rev(K)) on a wall face
+        if (pull_wall<Q, R>(s)) {
+            const int fl = flags[c + pull_offset<Q, R>(s)];
+            const T cub = (T)cx * bc.vel[fl][0] + (T)cy * bc.vel[fl][1] + (T)cz * bc.vel[fl][2];
+            const T term = (T)2 * w_scaled<Q, P>(K) * rho * cub * (T)3;
+            f[K] = f[K] - term;
+        }
+        wall_terms<Q, K + 1, P>(f, rho, flags, c, s, bc);
+    }
+}
+
 // Moments and, when `collide`, the BGK update with the velocity-shift force
 // in place (bgk.hpp).  Returns rho.
 template <int Q, class P>
@@ -128,5 +281,33 @@ template <int Q, int V>
 bool launch_pull_row(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts, bool nb,
                      const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
                      const uint32_t* nbmask, unsigned nblocks, hipStream_t stream);
+
+// Tiles of a two-step launch: ntx x nty tiles of 64 x BY cells in every
+// chunk of zc planes, nzc chunks; xcd = 1 puts y-stripe j of every chunk on
+// XCD j (nty % 8 == 0), so that tiles sharing their grown borders share an L2.
+struct TwoGeom {
+    uint32_t ntx, nty, nzc, zc;
+    uint32_t xcd;
+};
+
+// Rows per tile for a lattice and a storage width: the largest BY whose
+// ring (3 planes x 66 x (BY + 2) cells x (Q populations + a 4-byte mask))
+// fits the 160 KiB of LDS a gfx950 workgroup may hold, within 1024 threads;
+// 0 where no BY >= 3 fits.
+__host__ __device__ constexpr int two_rows(int q, int sbytes) {
+    int best = 0;
+    for (int by = 1; by <= 14; ++by)
+        if (3 * 66 * (by + 2) * (q * sbytes + 4) <= 160 * 1024 && 66 * (by + 2) <= 1024) best = by;
+    return best >= 3 ? best : 0;
+}
+__host__ __device__ constexpr int two_cells(int by) { return 66 * (by + 2); }
+__host__ __device__ constexpr int two_threads(int by) { return (two_cells(by) + 63) / 64 * 64; }
+
+// Two pull steps in one pass (fused_two.hpp), tiles of 64 x two_rows(Q, S)
+// cells.  Returns false where no kernel exists for (Q, V).
+template <int Q, int V>
+bool launch_pull2(const FusedArgs<typename Prec<V>::T>& a, const TwoGeom& g, bool nb, bool mov, bool nts,
+                  const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
+                  const uint32_t* nbmask, hipStream_t stream);
 
 }  // namespace plb

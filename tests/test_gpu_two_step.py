@@ -1,0 +1,159 @@
+"""Two pull steps per launch (PLB_TUNE_TWO_STEP): the numbers of the oracle's
+corrected step, and of the one-step pull kernels, bit for bit, for every
+storage variant the two-step kernel is built for; tiles cut ragged in y,
+several z-chunks, periodic seams on every axis, walls on every axis (with a
+moving lid), interior solids; even and odd step counts, reads between runs
+and graph replays; and at the full 256^3 size of the headline workload."""
+import numpy as np
+import pytest
+
+import plbpd_amd as P
+from plbpd_amd import capi
+from oracle.pyoracle import Boundary, NOSLIP, PERIODIC, EAST, WEST, NORTH, SOUTH, UP, DOWN
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = {19: ["f64", "f32", "f64a32", "f32s", "f16s"], 27: ["f32", "f32s", "f16s"]}
+
+# q, X, Y, Z, walls ("z": up/down, "y": north/south, "x": east/west, None), solids, z-chunk
+CASES = [
+    (19, 64, 12, 10, "z", True, 3),
+    (19, 64, 7, 9, "z", False, 0),
+    (19, 128, 5, 6, None, True, 4),
+    (19, 64, 10, 8, "y", True, 5),
+    (19, 64, 9, 7, "x", False, 2),
+    (19, 64, 1, 5, None, False, 1),
+    (27, 64, 11, 9, "z", True, 4),
+    (27, 64, 6, 6, "y", False, 0),
+]
+
+
+def bcs(walls, lid=(0.03, 0.0, 0.0)):
+    t = [PERIODIC] * 6
+    v = {}
+    if walls == "z":
+        t[UP] = t[DOWN] = NOSLIP
+        v = {UP: lid, DOWN: (0.0, 0.0, 0.0)}
+    elif walls == "y":
+        t[NORTH] = t[SOUTH] = NOSLIP
+        v = {NORTH: (0.02, 0.0, -0.01), SOUTH: (0.0, 0.0, 0.0)}
+    elif walls == "x":
+        t[EAST] = t[WEST] = NOSLIP
+        v = {EAST: (0.0, 0.02, 0.01), WEST: (0.0, 0.0, 0.0)}
+    return Boundary(type=t, vel=v), P.boundary(t, v)
+
+
+def two_step(st, on=True, zc=0):
+    st.set_tuning(capi.TUNE_TWO_STEP, 1 if on else 0)
+    st.set_tuning(capi.TUNE_TWO_ZCHUNK, zc)
+
+
+@pytest.mark.parametrize("q,X,Y,Z,walls,solids,zc", CASES)
+@pytest.mark.parametrize("steps", [12, 13])
+def test_two_step_matches_oracle(orc, q, X, Y, Z, walls, solids, zc, steps):
+    ob, pb = bcs(walls)
+    tau, F = 0.61, (2e-5, -1e-6, 1e-6)
+    mask = (np.random.default_rng(17).random((Z, Y, X)) < 0.1).astype(np.uint8) if solids else None
+    init = (P.INIT_PERTURBED, 5, 2e-3)
+    fluid = orc.cor_flags(q, X, Y, Z, ob, mask) == 0
+    for variant in VARIANTS[q]:
+        print(f"\n[case] two-step {variant} q={q} {X}x{Y}x{Z} walls={walls} solids={solids} zc={zc} "
+              f"steps={steps}", flush=True)
+        o = orc.cor_state(variant, q, X, Y, Z)
+        orc.cor_init(o, 1.0, (0.01, 0.0, 0.0), init)
+        orc.cor_run(o, tau, F, ob, steps, mask)
+        with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+            two_step(st, True, zc)
+            st.set_boundary(pb, mask)
+            st.initialize(1.0, (0.01, 0.0, 0.0), init, pb)
+            st.configure(tau, F, pb)
+            st.step(steps)
+            raw = st.download_raw()
+            rho, u = st.moments()
+        if variant == "f16s":
+            assert np.array_equal(raw.view(np.uint16)[fluid], o["f"].view(np.uint16)[fluid]), variant
+        else:
+            assert np.array_equal(raw[fluid], o["f"][fluid]), variant
+        orho, ou = orc.cor_moments(o, ob, mask)
+        assert np.array_equal(rho, orho) and np.array_equal(u, ou), variant
+
+
+@pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
+def test_two_step_equals_one_step_with_reads_and_graphs(orc, variant):
+    """Runs split by reads (each read gathers the pending streaming), graph
+    replays of 10 steps (four two-step launches and two one-step launches, so
+    that the graph keeps its buffer parity) and of 12 (six two-step launches),
+    and odd remainders."""
+    q, X, Y, Z = 19, 64, 13, 11
+    ob, pb = bcs("z", lid=(0.02, 0.0, 0.0))
+    mask = (np.random.default_rng(4).random((Z, Y, X)) < 0.08).astype(np.uint8)
+    fl = orc.cor_flags(q, X, Y, Z, ob, mask) == 0
+    with P.State(q, X, Y, Z, variant, P.FUSED) as a, P.State(q, X, Y, Z, variant, P.FUSED) as b:
+        two_step(a, True, 4)
+        two_step(b, False)
+        for st in (a, b):
+            st.set_boundary(pb, mask)
+            st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 9, 1e-3), pb)
+            st.configure(0.57, (1e-5, 0, 0), pb)
+        for chunk in (1, 2, 7, 4, 3):
+            a.step(chunk)
+            b.step(chunk)
+            assert np.array_equal(a.download_raw()[fl], b.download_raw()[fl]), chunk
+            ra, ua = a.moments()
+            rb, ub = b.moments()
+            assert np.array_equal(ra, rb) and np.array_equal(ua, ub), chunk
+        for spg in (10, 12, 10):
+            a.step_graph(2 * spg + 1, spg)
+            b.step(2 * spg + 1)
+            assert np.array_equal(a.download_raw()[fl], b.download_raw()[fl]), spg
+        assert np.array_equal(a.global_moments(), b.global_moments())
+
+
+def test_two_step_knobs():
+    with P.State(19, 64, 8, 8, "f32s", P.FUSED) as st:
+        with pytest.raises(P.InvalidArgument):
+            st.set_tuning(capi.TUNE_TWO_STEP, 2)
+        with pytest.raises(P.InvalidArgument):
+            st.set_tuning(capi.TUNE_TWO_ZCHUNK, -1)
+        st.set_tuning(capi.TUNE_TWO_STEP, -1)
+        st.set_tuning(capi.TUNE_TWO_ZCHUNK, 0)
+        assert st.scheme() == "pull2" and st.steps_per_launch() == 2
+        st.set_tuning(capi.TUNE_TWO_STEP, 0)
+        assert st.scheme() == "pull"
+    # automatic: 4-byte storage only; forced on where a kernel exists
+    with P.State(19, 64, 8, 8, "f16s", P.FUSED) as st:
+        assert st.scheme() == "pull"
+        st.set_tuning(capi.TUNE_TWO_STEP, 1)
+        assert st.scheme() == "pull2"
+    for q, x, v in ((19, 96, "f32s"), (9, 64, "f32s"), (27, 64, "f64")):
+        with P.State(q, x, 1 if q == 9 else 8, 8, v, P.FUSED) as st:
+            st.set_tuning(capi.TUNE_TWO_STEP, 1)
+            assert st.scheme() == "pull", (q, x, v)
+    with P.State(19, 64, 8, 8, "f32s", P.FUSED) as st:
+        st.set_tuning(capi.TUNE_SCHEME, capi.SCHEME_PUSH)
+        assert st.scheme() == "push"
+
+
+@pytest.mark.parametrize("variant,solids", [("f32s", True), ("f32s", False), ("f16s", True), ("f64", True)])
+def test_two_step_full_size(variant, solids):
+    """The headline lattice (D3Q19 256^3 channel, 10% solid voxels): two-step
+    launches equal one-step launches bit for bit after 20 steps (graph replays,
+    the launch geometry of the benchmark: eight XCD stripes, two z-chunks)."""
+    N = 256
+    bc = P.channel()
+    mask = (np.random.default_rng(20250303).random((N, N, N)) < 0.10).astype(np.uint8) if solids else None
+    print(f"\n[case] two-step full size {variant} solids={solids}", flush=True)
+    out = []
+    for on in (True, False):
+        with P.State(19, N, N, N, variant, P.FUSED) as st:
+            two_step(st, on)
+            st.set_boundary(bc, mask)
+            st.initialize(1.0, (0.0, 0.0, 0.0), (P.INIT_PERTURBED, 3, 1e-3), bc)
+            st.configure(0.55, (5e-7, 0.0, 0.0), bc)
+            st.step_graph(21, 20)
+            out.append((st.download_raw(), st.global_moments()))
+    fluid = np.ones((N, N, N), bool) if mask is None else mask == 0
+    fluid[0] = False
+    fluid[-1] = False
+    assert np.array_equal(out[0][0][fluid], out[1][0][fluid])
+    assert np.array_equal(out[0][1], out[1][1])
