@@ -300,9 +300,10 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
                               /* each XCD takes a y-stripe of every z-plane (rows */
                               /* % 8 == 0, else 0); -1 (default) = 2 for the cell */
                               /* kernels around interior solids, else 0           */
-#define PLB_TUNE_INDEX64 7    /* 1: the cell kernels' 64-bit index instantiations even */
-                              /* where 32-bit byte offsets fit (they run by themselves */
-                              /* for buffers over 4 GiB); default 0                    */
+#define PLB_TUNE_INDEX64 7    /* 1: the cell kernels' 64-bit index instantiations, and */
+                              /* the two-step kernel's per-population base pointers,   */
+                              /* even where 32-bit byte offsets fit (they run by       */
+                              /* themselves for buffers over 4 GiB); default 0         */
 #define PLB_TUNE_LITERAL_FUSE 8 /* literal mode: 1 (default) runs a collision and the */
                                 /* push that follows it as one kernel (same values in */
                                 /* n and ntmp, one pass fewer over n), and computes   */
@@ -316,9 +317,11 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
                                 /* kernel exists: D3Q19, and D3Q27 with 2- or 4-byte  */
                                 /* storage, x a multiple of 64, 32-bit offsets; 0     */
                                 /* one step per launch; -1 (default) automatic: on    */
-                                /* for 4-byte storage                                  */
+                                /* for 4- and 8-byte storage                           */
 #define PLB_TUNE_TWO_ZCHUNK 10  /* planes per z-chunk of a two-step launch (0: chosen */
                                 /* from the grid and the compute-unit count)          */
+#define PLB_TUNE_TWO_TILE_X 11  /* two-step tile width: 64 cells (one workgroup per   */
+                                /* compute unit) or 32 (two); 0 automatic             */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme the next fused steps will use: PLB_SCHEME_PUSH,
  * PLB_SCHEME_PULL, PLB_SCHEME_PULL_TWO (pull, two steps per launch, for

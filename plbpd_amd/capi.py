@@ -30,7 +30,7 @@ LAYOUT_REFERENCE, LAYOUT_COMPACT = 0, 1
 DTYPE_F64, DTYPE_F32 = 0, 1
 INIT_UNIFORM, INIT_PERTURBED, INIT_TAYLOR_GREEN = 0, 1, 2
 (TUNE_BLOCK_X, TUNE_NT_LOAD, TUNE_NT_STORE, TUNE_SCHEME, TUNE_TILE_ORDER, TUNE_VECTOR, TUNE_XCD, TUNE_INDEX64,
- TUNE_LITERAL_FUSE, TUNE_TWO_STEP, TUNE_TWO_ZCHUNK) = range(11)
+ TUNE_LITERAL_FUSE, TUNE_TWO_STEP, TUNE_TWO_ZCHUNK, TUNE_TWO_TILE_X) = range(12)
 SCHEME_AUTO, SCHEME_PUSH, SCHEME_PULL, SCHEME_AA, SCHEME_PULL_TWO = 0, 1, 2, 3, 4
 STAGES = ["collide", "stream", "noslip", "periodicity", "moments", "velocity_reset",
           "density_reset", "fused"]

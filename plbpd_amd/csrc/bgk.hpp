@@ -56,6 +56,18 @@ __device__ inline typename P::T w_scaled(int k) {
     return (typename P::T)Lat<Q>::w(k) * P::kScale;
 }
 
+// Directions 2j - 1 and 2j are opposite with equal weights (LBModel.cpp's
+// ordering), which the pair loop of bgk_update relies on.
+template <int Q>
+constexpr bool pairs_opposite() {
+    if (Q % 2 != 1) return false;
+    for (int k = 1; k < Q; k += 2)
+        if (Lat<Q>::cx(k + 1) != -Lat<Q>::cx(k) || Lat<Q>::cy(k + 1) != -Lat<Q>::cy(k) ||
+            Lat<Q>::cz(k + 1) != -Lat<Q>::cz(k) || Lat<Q>::w(k + 1) != Lat<Q>::w(k))
+            return false;
+    return true;
+}
+
 // Moments and, when `collide`, the BGK update in place, on f in the storage
 // scale (P::load_raw: kScale * f, kScale a power of two).  Returns rho.
 // Shifted storage sums deviations and adds 1 afterwards; its equilibrium is
@@ -67,6 +79,7 @@ __device__ inline typename P::T bgk_update(typename P::T (&f)[Q], typename P::T 
                                            typename P::T omm, typename P::T F0, typename P::T F1,
                                            typename P::T F2, bool collide) {
     using T = typename P::T;
+    static_assert(pairs_opposite<Q>(), "bgk_update pairs direction k with k + 1 = rev(k)");
     T r, j0, j1, j2;
     moment_sums<Q, T>(f, r, j0, j1, j2);
     if (P::kScale != (T)1) {
@@ -85,16 +98,35 @@ __device__ inline typename P::T bgk_update(typename P::T (&f)[Q], typename P::T 
     const T u1 = j1 * ri + F1 * tri;
     const T u2 = j2 * ri + F2 * tri;
     const T usq = u0 * u0 + u1 * u1 + u2 * u2;
+    const T u15 = (T)1.5 * usq;
+    // rest direction: c_0 . u = 0
+    {
+        const T w = w_scaled<Q, P>(0);
+        const T cu = c_dot<Q, T>(0, u0, u1, u2);
+        const T t3 = (T)3 * cu, q = ((T)4.5 * cu) * cu;
+        if (P::shifted)
+            f[0] = omm * f[0] + om * (w * (drho + r * ((t3 + q) - u15)));
+        else
+            f[0] = omm * f[0] + om * ((w * r) * ((((T)1 + t3) + q) - u15));
+    }
+    // Directions k and k + 1 = rev(k) are opposite, with equal weights.  Each
+    // rounding is symmetric under negation, so c_{k+1}.u == -(c_k.u),
+    // 3 c_{k+1}.u == -(3 c_k.u) and (4.5 c_{k+1}.u) c_{k+1}.u == (4.5 c_k.u) c_k.u
+    // exactly (a zero c.u gives +0 sums either way): the pair shares its
+    // products, and every result is the one the per-direction expression
+    // (3 cu + 4.5 cu cu) - 1.5 u^2 gives.
 #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const T cu = c_dot<Q, T>(k, u0, u1, u2);
+    for (int k = 1; k < Q; k += 2) {
         const T w = w_scaled<Q, P>(k);
+        const T cu = c_dot<Q, T>(k, u0, u1, u2);
+        const T t3 = (T)3 * cu, q = ((T)4.5 * cu) * cu;
         if (P::shifted) {
-            const T geq = w * (drho + r * ((((T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq));
-            f[k] = omm * f[k] + om * geq;
+            f[k] = omm * f[k] + om * (w * (drho + r * ((t3 + q) - u15)));
+            f[k + 1] = omm * f[k + 1] + om * (w * (drho + r * ((q - t3) - u15)));
         } else {
-            const T neq = (w * r) * ((((T)1 + (T)3 * cu) + ((T)4.5 * cu) * cu) - (T)1.5 * usq);
-            f[k] = omm * f[k] + om * neq;
+            const T wr = w * r;
+            f[k] = omm * f[k] + om * (wr * ((((T)1 + t3) + q) - u15));
+            f[k + 1] = omm * f[k + 1] + om * (wr * ((((T)1 - t3) + q) - u15));
         }
     }
     return r;

@@ -181,6 +181,11 @@ int plb_set_tuning(plb_state* s, int key, int value) {
             if (value < -1 || value > 1) return fail(PLB_ERR_INVALID_ARGUMENT, "two-step setting must be -1, 0 or 1");
             s->tune_two = value;
             break;
+        case PLB_TUNE_TWO_TILE_X:
+            if (value != 0 && value != 32 && value != 64)
+                return fail(PLB_ERR_INVALID_ARGUMENT, "two-step tile width must be 0, 32 or 64");
+            s->tune_two_bx = value;
+            break;
         case PLB_TUNE_TWO_ZCHUNK:
             if (value < 0) return fail(PLB_ERR_INVALID_ARGUMENT, "z-chunk length must be 0 (automatic) or positive");
             s->tune_two_zc = value;

@@ -293,6 +293,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->tune_lit_fuse = 1;
     s->tune_two = -1;
     s->tune_two_zc = 0;
+    s->tune_two_bx = 0;
     s->num_cu = 256;
     {
         int cu = 0;

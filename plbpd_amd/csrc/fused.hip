@@ -668,11 +668,15 @@ enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly, kAaEven, kAa
 // zc planes costs zc + 2 planes of work (the grown planes at its ends): the
 // chunk count minimises rounds x (zc + 2).
 TwoGeom two_geom(const plb_state* s) {
-    const int by = two_rows(s->q, (int)s->sbytes);
+    const int bx = fused_two_tile_x(s);
+    const int by = two_rows(s->q, (int)s->sbytes, bx);
     TwoGeom g;
-    g.ntx = (uint32_t)s->X / 64u;
+    g.bx = (uint32_t)bx;
+    g.wide = (s->tune_index64 || (uint64_t)s->q * s->pstride * s->sbytes > 0xffffffffull) ? 1u : 0u;
+    g.ntx = (uint32_t)s->X / (uint32_t)bx;
     g.nty = ((uint32_t)s->Y + (uint32_t)by - 1u) / (uint32_t)by;
-    const uint64_t tiles = (uint64_t)g.ntx * g.nty, Z = (uint64_t)s->Z, ncu = (uint64_t)s->num_cu;
+    const uint64_t tiles = (uint64_t)g.ntx * g.nty, Z = (uint64_t)s->Z;
+    const uint64_t ncu = (uint64_t)s->num_cu * (uint64_t)two_wgs(bx);  // workgroups resident at once
     uint64_t zc = Z;
     if (s->tune_two_zc > 0) {
         zc = std::min<uint64_t>((uint64_t)s->tune_two_zc, Z);
@@ -796,8 +800,8 @@ int launch_kind(plb_state* s, KernelKind kind) {
                 if constexpr (Q == 9) {
                     return fail(PLB_ERR_LOGIC, "no two-step kernel for D2Q9");
                 } else {
-                    if (!launch_pull2<Q, V>(a, two_geom(s), nb, a.bc.moving != 0, a.nt_store != 0, A, B, s->flags,
-                                            s->nbmask, s->stream))
+                    if (!launch_pull2<Q, V>(a, two_geom(s), nb, a.bc.moving != 0, A, B, s->flags, s->nbmask,
+                                            s->stream))
                         return fail(PLB_ERR_LOGIC, "no two-step kernel for this storage");
                 }
                 break;
@@ -893,17 +897,31 @@ int launch_kind(plb_state* s, KernelKind kind) {
 
 }  // namespace
 
+int fused_two_tile_x(const plb_state* s) {
+    // arrays beyond 4 GiB (or PLB_TUNE_INDEX64) take per-population base
+    // pointers, built for 64-cell tiles
+    const bool wide = s->tune_index64 || (uint64_t)s->q * s->pstride * s->sbytes > 0xffffffffull;
+    auto fits = [&](int bx) {
+        return s->X % bx == 0 && two_rows(s->q, (int)s->sbytes, bx) > 0 && (!wide || bx == 64);
+    };
+    if (s->tune_two_bx > 0) return fits(s->tune_two_bx) ? s->tune_two_bx : 0;
+    for (int bx : {64, 32})
+        if (fits(bx)) return bx;
+    return 0;
+}
+
 bool fused_two_eligible(const plb_state* s) {
     if (s->tune_two == 0 || s->aa || !fused_uses_pull(s) || s->mode != PLB_MODE_FUSED) return false;
-    // automatic: 4-byte storage, where two steps per pass measured 1.25x
-    // (D3Q19 256^3 FP32-shifted, 10% solids) to 1.5x (walls only) faster on
-    // MI355X; 2-byte storage ran slower (its step is closer to the
-    // arithmetic limit) and 8-byte storage slower with solids (3-row tiles)
-    if (s->tune_two < 0 && s->sbytes != 4) return false;
-    if ((s->q != 19 && s->q != 27) || two_rows(s->q, (int)s->sbytes) == 0) return false;
-    if (s->X % 64 != 0 || s->tune_index64) return false;
-    // the kernel's offsets are 32-bit
-    return (uint64_t)s->q * s->pstride * s->sbytes <= 0xffffffffull;
+    // automatic: 4- and 8-byte storage, where two steps per pass measured
+    // 1.28x (D3Q19 256^3 FP32-shifted, 10% solids), 1.56x (walls only), 1.07x
+    // and 1.39x (FP64) and 1.35x (D3Q27 FP32-shifted, walls) faster on MI355X;
+    // 2-byte storage ran slower (1.34x with solids: its one-step kernel is
+    // closer to the arithmetic limit)
+    if (s->tune_two < 0 && s->sbytes == 2) return false;
+    if ((s->q != 19 && s->q != 27) || fused_two_tile_x(s) == 0) return false;
+    // offsets within one population array are 32-bit
+    if ((uint64_t)s->pstride * s->sbytes > 0xffffffffull) return false;
+    return true;
 }
 
 int fused_launch_two(plb_state* s) {

@@ -282,31 +282,73 @@ bool launch_pull_row(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts,
                      const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
                      const uint32_t* nbmask, unsigned nblocks, hipStream_t stream);
 
-// Tiles of a two-step launch: ntx x nty tiles of 64 x BY cells in every
+// Tiles of a two-step launch: ntx x nty tiles of bx x BY cells in every
 // chunk of zc planes, nzc chunks; xcd = 1 puts y-stripe j of every chunk on
 // XCD j (nty % 8 == 0), so that tiles sharing their grown borders share an L2.
 struct TwoGeom {
     uint32_t ntx, nty, nzc, zc;
     uint32_t xcd;
+    uint32_t bx;    // 64: one workgroup per compute unit; 32: two
+    uint32_t wide;  // population arrays beyond 4 GiB: per-population base pointers (bx 64 only)
 };
 
-// Rows per tile for a lattice and a storage width: the largest BY whose
-// ring (3 planes x 66 x (BY + 2) cells x (Q populations + a 4-byte mask))
-// fits the 160 KiB of LDS a gfx950 workgroup may hold, within 1024 threads;
-// 0 where no BY >= 3 fits.
-__host__ __device__ constexpr int two_rows(int q, int sbytes) {
+// The step t + 1 ring of the two-step kernel.  Directions are grouped by
+// c_z (+1, 0, -1).  The step t + 2 cell at plane z pulls the c_z = +1 group
+// from plane z - 1, the c_z = 0 group from plane z and the c_z = -1 group
+// from plane z + 1, and its own reverse slots (bounce-back) from plane z: a
+// plane's c_z = +1 group is read up to two planes after it is computed, the
+// other groups up to one, so the groups keep 3, 2 and 2 planes.
+template <int Q>
+struct ZRing {
+    __host__ __device__ static constexpr int grp(int k) {
+        return Lat<Q>::cz(k) > 0 ? 0 : (Lat<Q>::cz(k) == 0 ? 1 : 2);
+    }
+    __host__ __device__ static constexpr int size(int g) {
+        int n = 0;
+        for (int k = 0; k < Q; ++k) n += grp(k) == g;
+        return n;
+    }
+    __host__ __device__ static constexpr int idx(int k) {
+        int n = 0;
+        for (int j = 0; j < k; ++j) n += grp(j) == grp(k);
+        return n;
+    }
+    __host__ __device__ static constexpr int slots(int g) { return g == 0 ? 3 : 2; }
+    __host__ __device__ static constexpr int base(int g) {
+        int b = 0;
+        for (int h = 0; h < g; ++h) b += size(h) * slots(h);
+        return b;
+    }
+    // population planes the ring holds per cell (43 for D3Q19, 63 for D3Q27)
+    __host__ __device__ static constexpr int planes() { return base(3); }
+};
+__host__ __device__ constexpr int zring_planes(int q) {
+    return q == 19 ? ZRing<19>::planes() : (q == 27 ? ZRing<27>::planes() : (q == 9 ? ZRing<9>::planes() : 0));
+}
+
+// Cells of a grown tile, and threads of its workgroup (one cell each).
+__host__ __device__ constexpr int two_cells(int bx, int by) { return (bx + 2) * (by + 2); }
+__host__ __device__ constexpr int two_threads(int bx, int by) { return (two_cells(bx, by) + 63) / 64 * 64; }
+// Workgroups per compute unit the LDS budget of a tile width allows.
+__host__ __device__ constexpr int two_wgs(int bx) { return bx == 32 ? 2 : 1; }
+// Rows per tile for a lattice, a storage width and a tile width: the largest
+// BY whose ring ((bx + 2) x (BY + 2) cells x (zring_planes(q) population
+// planes + two 4-byte mask planes)) fits the LDS share of a workgroup
+// (160 KiB per gfx950 compute unit, split between two_wgs(bx) workgroups),
+// within 1024 threads; 0 where no BY >= 3 fits.
+__host__ __device__ constexpr int two_rows(int q, int sbytes, int bx) {
     int best = 0;
-    for (int by = 1; by <= 14; ++by)
-        if (3 * 66 * (by + 2) * (q * sbytes + 4) <= 160 * 1024 && 66 * (by + 2) <= 1024) best = by;
+    for (int by = 1; by <= 30; ++by)
+        if (two_cells(bx, by) * (zring_planes(q) * sbytes + 8) <= 160 * 1024 / two_wgs(bx) &&
+            two_cells(bx, by) <= 1024)
+            best = by;
     return best >= 3 ? best : 0;
 }
-__host__ __device__ constexpr int two_cells(int by) { return 66 * (by + 2); }
-__host__ __device__ constexpr int two_threads(int by) { return (two_cells(by) + 63) / 64 * 64; }
 
-// Two pull steps in one pass (fused_two.hpp), tiles of 64 x two_rows(Q, S)
-// cells.  Returns false where no kernel exists for (Q, V).
+// Two pull steps in one pass (fused_two.hpp), tiles of g.bx x two_rows(Q, S,
+// g.bx) cells.  Returns false where no kernel exists for (Q, V, g.bx, g.wide).
 template <int Q, int V>
-bool launch_pull2(const FusedArgs<typename Prec<V>::T>& a, const TwoGeom& g, bool nb, bool mov, bool nts,
+bool launch_pull2(const FusedArgs<typename Prec<V>::T>& a, const TwoGeom& g, bool nb, bool mov,
                   const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
                   const uint32_t* nbmask, hipStream_t stream);
 

@@ -5,11 +5,12 @@
 // Boundary.cpp:189-221), bit for bit, from one read of the populations of
 // step t and one write of those of step t + 2.
 //
-// A workgroup owns a tile of 64 x-cells by BY y-rows and sweeps it along z
+// A workgroup owns a tile of BX x-cells by BY y-rows and sweeps it along z
 // over a chunk of planes.  At each plane zi it computes step t + 1 on the
-// tile grown by one cell in x and y (66 x (BY + 2) cells, one per thread),
-// gathering step t from HBM, and keeps the result in LDS, in a ring of three
-// planes.  It then computes step t + 2 on the tile at plane zi - 1, gathering
+// tile grown by one cell in x and y ((BX + 2) x (BY + 2) cells, one per
+// thread), gathering step t from HBM, and keeps the result in an LDS ring
+// (ZRing: the directions with c_z = +1 for three planes, the others for
+// two).  It then computes step t + 2 on the tile at plane zi - 1, gathering
 // from the ring (planes zi - 2, zi - 1, zi), and writes it to the other
 // population array.  The grown border is computed by both neighbouring tiles,
 // so workgroups never wait for each other: every workgroup reads only A and
@@ -29,36 +30,102 @@ namespace plb {
 
 namespace {
 
+// Element (population k, cell offset i) of a population array.  W = false:
+// one 32-bit element offset k * pstride + i from the array base (every byte
+// offset of the array fits 32 bits); W = true: the population's base pointer
+// (a 64-bit scalar) plus the 32-bit cell offset (every byte offset within one
+// population fits 32 bits), for arrays beyond 4 GiB.
+template <bool W, class S>
+__device__ inline S* pop_elem(S* A, uint32_t k, uint32_t ps, uint32_t i) {
+    if constexpr (W)
+        return elem(A + (uint64_t)k * ps, i);
+    else
+        return elem(A, k * ps + i);
+}
+
 // Step t loads without interior solids: population k from x - c_k, or the
 // cell's own slot rev(k) where x - c_k is a wall layer (known from the
 // coordinates, so no load waits on a flag).
-template <int Q, int K, int V, class I>
-__device__ inline void pull_gather_walls(typename Prec<V>::S (&fv)[Q], const typename Prec<V>::S* __restrict__ A,
-                                         I c, I ps, const Seams<I>& s) {
+template <int Q, int K, int V, bool W>
+__device__ inline void gather_walls(typename Prec<V>::S (&fv)[Q], const typename Prec<V>::S* __restrict__ A,
+                                    uint32_t c, uint32_t ps, const Seams<uint32_t>& s) {
     if constexpr (K < Q) {
-        const I moved = (I)K * ps + (c + pull_offset<Q, K>(s));
-        const I own = (I)rev_k<Q>(K) * ps + c;
-        fv[K] = *elem(A, pull_wall<Q, K>(s) ? own : moved);
-        pull_gather_walls<Q, K + 1, V>(fv, A, c, ps, s);
+        const bool wall = pull_wall<Q, K>(s);
+        fv[K] = *pop_elem<W>(A, wall ? (uint32_t)rev_k<Q>(K) : (uint32_t)K, ps,
+                             wall ? c : c + pull_offset<Q, K>(s));
+        gather_walls<Q, K + 1, V, W>(fv, A, c, ps, s);
     }
 }
 
-// Step t + 1 values for the step t + 2 cell at ring index li: population k
-// from li - c_k in the plane zo - c_k,z (ring slots sm1, s0, sp for planes
-// zo - 1, zo, zo + 1), or the cell's own slot rev(k) in plane zo where
-// x - c_k is not fluid.
-template <int Q, int K, int V, int NB, int NI, int RX, class I>
-__device__ inline void ring_gather(typename Prec<V>::T (&f)[Q], const typename Prec<V>::S* ring, I li, uint32_t sp,
-                                   uint32_t s0, uint32_t sm1, uint32_t m, const Seams<I>& s) {
+// Step t loads around interior solids: population k from x - c_k, or the
+// cell's own slot rev(k) where the neighbour mask m says x - c_k is not fluid
+// (one load per population; the mask is loaded an iteration ahead).
+template <int Q, int K, int V, bool W>
+__device__ inline void gather_mask(typename Prec<V>::S (&fv)[Q], const typename Prec<V>::S* __restrict__ A,
+                                   uint32_t c, uint32_t ps, uint32_t m, const Seams<uint32_t>& s) {
     if constexpr (K < Q) {
+        constexpr uint32_t R = (uint32_t)rev_k<Q>(K);
+        const bool solid = ((m >> R) & 1u) != 0;
+        fv[K] = *pop_elem<W>(A, solid ? R : (uint32_t)K, ps, solid ? c : c + pull_offset<Q, K>(s));
+        gather_mask<Q, K + 1, V, W>(fv, A, c, ps, m, s);
+    }
+}
+
+// Step t + 2 of a cell into its own slots (zeros for a cell that is not fluid).
+template <int Q, int V, bool NT, bool W>
+__device__ inline void store_own(const typename Prec<V>::T (&f)[Q], typename Prec<V>::S* __restrict__ B, uint32_t c,
+                                 uint32_t ps, bool zero) {
+    using P = Prec<V>;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const typename P::S v = P::store_raw(zero ? (typename P::T)0 : f[k]);
+        typename P::S* p = pop_elem<W>(B, (uint32_t)k, ps, c);
+        if (NT)
+            __builtin_nontemporal_store(v, p);
+        else
+            *p = v;
+    }
+}
+
+// Ring plane of direction k in its group's slot: element offset of cell 0.
+template <int Q, int K, int NI>
+__device__ inline uint32_t ring_at(uint32_t slot) {
+    using Z = ZRing<Q>;
+    constexpr int g = Z::grp(K);
+    return ((uint32_t)Z::base(g) + slot * (uint32_t)Z::size(g) + (uint32_t)Z::idx(K)) * (uint32_t)NI;
+}
+
+// Step t + 1 of a cell into the ring: direction k to its group's slot for
+// this plane (a3 = plane mod 3 for the c_z = +1 group, a2 = plane mod 2 for
+// the others).
+template <int Q, int K, int V, int NI>
+__device__ inline void ring_store(const typename Prec<V>::T (&f)[Q], typename Prec<V>::S* ring, uint32_t cell,
+                                  uint32_t a3, uint32_t a2) {
+    if constexpr (K < Q) {
+        ring[ring_at<Q, K, NI>(ZRing<Q>::grp(K) == 0 ? a3 : a2) + cell] = Prec<V>::store_raw(f[K]);
+        ring_store<Q, K + 1, V, NI>(f, ring, cell, a3, a2);
+    }
+}
+
+// Step t + 1 values for the step t + 2 cell at ring index li (plane zo):
+// population k from li - c_k in plane zo - c_k,z, or the cell's own slot
+// rev(k) in plane zo where x - c_k is not fluid.  Slots: pa = the c_z = +1
+// group's slot of plane zo - 1, oa = its slot of plane zo, b0 = the other
+// groups' slot of plane zo, c1 = the c_z = -1 group's slot of plane zo + 1.
+template <int Q, int K, int V, int NB, int NI, int RX>
+__device__ inline void ring_gather(typename Prec<V>::T (&f)[Q], const typename Prec<V>::S* ring, uint32_t li,
+                                   uint32_t pa, uint32_t oa, uint32_t b0, uint32_t c1, uint32_t m,
+                                   const Seams<uint32_t>& s) {
+    if constexpr (K < Q) {
+        using Z = ZRing<Q>;
         constexpr int R = rev_k<Q>(K);
-        constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
+        constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K);
+        constexpr int g = Z::grp(K), gr = Z::grp(R);
         const bool solid = NB ? ((m >> R) & 1u) != 0 : pull_wall<Q, K>(s);
-        const uint32_t src = cz > 0 ? sm1 : (cz < 0 ? sp : s0);
-        const I moved = (src * (uint32_t)Q + (uint32_t)K) * (uint32_t)NI + li - (I)(cx + cy * RX);
-        const I own = (s0 * (uint32_t)Q + (uint32_t)R) * (uint32_t)NI + li;
+        const uint32_t moved = ring_at<Q, K, NI>(g == 0 ? pa : (g == 1 ? b0 : c1)) + li - (uint32_t)(cx + cy * RX);
+        const uint32_t own = ring_at<Q, R, NI>(gr == 0 ? oa : b0) + li;
         f[K] = Prec<V>::load_raw(ring[solid ? own : moved]);
-        ring_gather<Q, K + 1, V, NB, NI, RX>(f, ring, li, sp, s0, sm1, m, s);
+        ring_gather<Q, K + 1, V, NB, NI, RX>(f, ring, li, pa, oa, b0, c1, m, s);
     }
 }
 
@@ -73,8 +140,8 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int Q, int V, int NB, bool MOV, bool NTS, int BY>
-__global__ __launch_bounds__(two_threads(BY)) void k_pull2(const typename Prec<V>::S* __restrict__ A,
+template <int Q, int V, int NB, bool MOV, int BX, int BY, bool W>
+__global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(const typename Prec<V>::S* __restrict__ A,
                                                            typename Prec<V>::S* __restrict__ B,
                                                            const uint8_t* __restrict__ flags,
                                                            const uint32_t* __restrict__ nbmask,
@@ -83,9 +150,9 @@ __global__ __launch_bounds__(two_threads(BY)) void k_pull2(const typename Prec<V
     using S = typename P::S;
     using T = typename P::T;
     using I = uint32_t;
-    constexpr int RX = 66, NI = two_cells(BY);
-    __shared__ S ring[3 * Q * NI];
-    __shared__ uint32_t mring[3 * NI];  // the step t + 1 cells' masks or flags
+    constexpr int RX = BX + 2, NI = two_cells(BX, BY);
+    __shared__ S ring[ZRing<Q>::planes() * NI];
+    __shared__ uint32_t mring[2 * NI];  // the step t + 1 cells' masks or flags, two planes
 
     const uint32_t tid = threadIdx.x;
     uint32_t tx, ty, tz;
@@ -103,7 +170,7 @@ __global__ __launch_bounds__(two_threads(BY)) void k_pull2(const typename Prec<V
     }
     const uint32_t X = a.X, Y = a.Y, Z = a.Z;
     const I XY = X * Y, ps = (I)a.pstride;
-    const uint32_t x0 = tx * 64u, y0 = ty * (uint32_t)BY, z0 = tz * g.zc;
+    const uint32_t x0 = tx * (uint32_t)BX, y0 = ty * (uint32_t)BY, z0 = tz * g.zc;
     const uint32_t nplanes = std::min(g.zc, Z - z0) + 2u;
 
     // step t + 1: this thread's cell of the grown tile (wrapped coordinates)
@@ -111,104 +178,132 @@ __global__ __launch_bounds__(two_threads(BY)) void k_pull2(const typename Prec<V
     const uint32_t ix = tid % RX, iy = tid / RX;
     const uint32_t gx = (x0 + ix + X - 1u) % X, gy = (y0 + iy + Y - 1u) % Y;
     // step t + 2: this thread's cell of the tile
-    const uint32_t ox = tid & 63u, oy = tid >> 6;
+    const uint32_t ox = tid % (uint32_t)BX, oy = tid / (uint32_t)BX;
     const bool in_o = oy < (uint32_t)BY && y0 + oy < Y;
     const uint32_t px = x0 + ox, py = y0 + oy;
     const I li = (oy + 1u) * RX + ox + 1u;
 
-    // loads of step t for the plane of iteration n (prefetched one iteration ahead)
-    S fv[Q], ow[Q];
-    uint32_t mi = 0;  // neighbour mask (NB) or flag of the step t + 1 cell
-    auto fetch = [&](uint32_t n) {
+    // Loads of step t for the plane of iteration n, issued one iteration
+    // ahead.  Around interior solids the gather addresses depend on the
+    // neighbour mask, which is loaded at the start of the iteration before,
+    // so that it has arrived when the gather is issued.
+    S fv[Q];
+    // neighbour mask (NB) or flag of this thread's step t + 1 cell in plane n
+    // Every thread loads, also past the grown tile (its wrapped coordinates
+    // stay in the lattice; the values are not used): loads under a branch
+    // made the compiler wait for loads in flight where the branches merge.
+    auto load_mask = [&](uint32_t n) -> uint32_t {
+        const I c = ((z0 + n + Z - 1u) % Z) * XY + gy * X + gx;
+        return NB ? nbmask[c] : (uint32_t)flags[c];
+    };
+    auto fetch = [&](uint32_t n, uint32_t m) {
         const uint32_t zi = (z0 + n + Z - 1u) % Z;
-        if (in_i) {
-            const I c = zi * XY + gy * X + gx;
-            const Seams<I> sm = seams<I>(gx, gy, zi, a);
-            if (NB) {
-                mi = nbmask[c];
-                pull_gather_both<Q, 0, V>(fv, ow, A, c, ps, sm);
-            } else {
-                mi = flags[c];
-                pull_gather_walls<Q, 0, V>(fv, A, c, ps, sm);
-            }
-        }
+        const I c = zi * XY + gy * X + gx;
+        const Seams<I> sm = seams<I>(gx, gy, zi, a);
+        if (NB)
+            gather_mask<Q, 0, V, W>(fv, A, c, ps, m, sm);
+        else
+            gather_walls<Q, 0, V, W>(fv, A, c, ps, sm);
     };
 
-    fetch(0);
-    for (uint32_t n = 0; n < nplanes; ++n) {
-        const uint32_t slot = n % 3u;
+    // One plane: the step t + 1 cell of plane n from the loads in fv and the
+    // mask mcur; the mask of plane n + 1 into mnext and its loads into fv;
+    // the step t + 2 cell of plane n - 1.  The loop runs it twice per trip
+    // with the two mask registers swapped, so that no register that a load
+    // in flight writes is copied (a copy would wait for the load).
+    auto plane = [&](uint32_t n, uint32_t& mcur, uint32_t& mnext) {
+        // (plane numbers wrap modulo Z, so the loads past the chunk's last
+        // plane stay in bounds; their values are not used)
+        mnext = load_mask(n + 1u);
+        // ring slots of plane n (relative plane numbers: only n mod 3 and mod 2 matter)
+        const uint32_t a3 = n % 3u, a2 = n & 1u;
         // step t + 1 at plane zi into the ring
         if (in_i) {
-            mring[slot * NI + tid] = mi;
-            const bool self_solid = NB ? (mi & 1u) != 0 : mi != 0;
+            mring[a2 * NI + tid] = mcur;
+            const bool self_solid = NB ? (mcur & 1u) != 0 : mcur != 0;
             if (!self_solid) {
                 T f[Q];
 #pragma unroll
-                for (int k = 0; k < Q; ++k)
-                    f[k] = P::load_raw(NB && k > 0 && ((mi >> rev_k<Q>(k)) & 1u) ? ow[k] : fv[k]);
+                for (int k = 0; k < Q; ++k) f[k] = P::load_raw(fv[k]);
                 const T rho = bgk<Q, P>(f, a, true);
                 if (MOV && a.bc.moving) {
                     const uint32_t zi = (z0 + n + Z - 1u) % Z;
                     const I c = zi * XY + gy * X + gx;
                     wall_terms<Q, 1, P>(f, rho, flags, c, seams<I>(gx, gy, zi, a), a.bc);
                 }
-#pragma unroll
-                for (int k = 0; k < Q; ++k) ring[(slot * Q + k) * NI + tid] = P::store_raw(f[k]);
+                ring_store<Q, 0, V, NI>(f, ring, tid, a3, a2);
             }
         }
-        if (n + 1u < nplanes) fetch(n + 1u);
+        fetch(n + 1u, mnext);
         lds_barrier();
         // step t + 2 at plane zo = zi - 1 from the ring
         if (n >= 2u && in_o) {
             const uint32_t zo = z0 + n - 2u;
             const I c = zo * XY + py * X + px;
-            const uint32_t sp = slot, s0 = (n + 2u) % 3u, sm1 = (n + 1u) % 3u;
+            // planes zo - 1, zo, zo + 1 are the planes of iterations n - 2, n - 1, n
+            const uint32_t pa = (n + 1u) % 3u, oa = (n + 2u) % 3u, b0 = (n + 1u) & 1u, c1 = a2;
             // this cell was a step t + 1 cell of the previous plane
-            const uint32_t mcur = mring[s0 * NI + li];
-            const bool self_solid = NB ? (mcur & 1u) != 0 : mcur != 0;
+            const uint32_t mo = mring[b0 * NI + li];
+            const bool self_solid = NB ? (mo & 1u) != 0 : mo != 0;
             if (!__all(self_solid)) {
                 const Seams<I> sm = seams<I>(px, py, zo, a);
                 T f[Q];
-                ring_gather<Q, 0, V, NB, NI, RX>(f, ring, li, sp, s0, sm1, mcur, sm);
+                ring_gather<Q, 0, V, NB, NI, RX>(f, ring, li, pa, oa, b0, c1, mo, sm);
                 if (!self_solid) {
                     const T rho = bgk<Q, P>(f, a, true);
                     if (MOV && a.bc.moving) wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
                 }
-                own_store<Q, V, NTS>(f, B, c, ps, self_solid);
+                if (a.nt_store)
+                    store_own<Q, V, true, W>(f, B, c, ps, self_solid);
+                else
+                    store_own<Q, V, false, W>(f, B, c, ps, self_solid);
             }
         }
         lds_barrier();
+    };
+
+    uint32_t m0 = load_mask(0), m1 = 0;
+    fetch(0, m0);
+    for (uint32_t n = 0; n < nplanes; n += 2u) {
+        plane(n, m0, m1);
+        if (n + 1u < nplanes) plane(n + 1u, m1, m0);
     }
 }
 
 }  // namespace
 
-template <int Q, int V>
-bool launch_pull2(const FusedArgs<typename Prec<V>::T>& a, const TwoGeom& g, bool nb, bool mov, bool nts,
-                  const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
-                  const uint32_t* nbmask, hipStream_t stream) {
-    constexpr int BY = two_rows(Q, (int)sizeof(typename Prec<V>::S));
+template <int Q, int V, int BX, bool W>
+bool launch_pull2_bx(const FusedArgs<typename Prec<V>::T>& a, const TwoGeom& g, bool nb, bool mov,
+                     const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
+                     const uint32_t* nbmask, hipStream_t stream) {
+    constexpr int BY = two_rows(Q, (int)sizeof(typename Prec<V>::S), BX);
     if constexpr (BY == 0) {
         return false;
     } else {
-        const dim3 grid(g.ntx * g.nty * g.nzc), block(two_threads(BY));
-#define PLB_TWO(NBv, MVv, NTv) \
-    hipLaunchKernelGGL((k_pull2<Q, V, NBv, MVv, NTv, BY>), grid, block, 0, stream, A, B, flags, nbmask, a, g)
-        if (nb && mov && nts) PLB_TWO(1, true, true);
-        else if (nb && mov) PLB_TWO(1, true, false);
-        else if (nb && nts) PLB_TWO(1, false, true);
-        else if (nb) PLB_TWO(1, false, false);
-        else if (mov && nts) PLB_TWO(0, true, true);
-        else if (mov) PLB_TWO(0, true, false);
-        else if (nts) PLB_TWO(0, false, true);
-        else PLB_TWO(0, false, false);
+        const dim3 grid(g.ntx * g.nty * g.nzc), block(two_threads(BX, BY));
+#define PLB_TWO(NBv, MVv) \
+    hipLaunchKernelGGL((k_pull2<Q, V, NBv, MVv, BX, BY, W>), grid, block, 0, stream, A, B, flags, nbmask, a, g)
+        if (nb && mov) PLB_TWO(1, true);
+        else if (nb) PLB_TWO(1, false);
+        else if (mov) PLB_TWO(0, true);
+        else PLB_TWO(0, false);
 #undef PLB_TWO
         return true;
     }
 }
 
+template <int Q, int V>
+bool launch_pull2(const FusedArgs<typename Prec<V>::T>& a, const TwoGeom& g, bool nb, bool mov,
+                  const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
+                  const uint32_t* nbmask, hipStream_t stream) {
+    if (g.bx == 64 && !g.wide) return launch_pull2_bx<Q, V, 64, false>(a, g, nb, mov, A, B, flags, nbmask, stream);
+    if (g.bx == 64) return launch_pull2_bx<Q, V, 64, true>(a, g, nb, mov, A, B, flags, nbmask, stream);
+    if (g.bx == 32 && !g.wide) return launch_pull2_bx<Q, V, 32, false>(a, g, nb, mov, A, B, flags, nbmask, stream);
+    return false;
+}
+
 #define PLB_TWO_INST(Q, V)                                                                                  \
-    template bool launch_pull2<Q, V>(const FusedArgs<typename Prec<V>::T>&, const TwoGeom&, bool, bool, bool, \
+    template bool launch_pull2<Q, V>(const FusedArgs<typename Prec<V>::T>&, const TwoGeom&, bool, bool,       \
                                      const typename Prec<V>::S*, typename Prec<V>::S*, const uint8_t*,       \
                                      const uint32_t*, hipStream_t);
 #define PLB_TWO_INST_Q(Q) \

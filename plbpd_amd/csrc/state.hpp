@@ -102,8 +102,9 @@ struct plb_state {
     int tune_index64;     // 1: 64-bit index kernels even where 32-bit offsets fit
     int tune_vector;      // row pull kernel cells per lane (1, 2, 4) where X allows; 0 off; -1 auto
     int tune_lit_fuse;    // literal: 1 collision and push in one kernel when they follow each other
-    int tune_two;         // fused pull: two steps per launch where a kernel exists; 0 off, 1 on, -1 auto (4-byte storage)
+    int tune_two;         // fused pull: two steps per launch where a kernel exists; 0 off, 1 on, -1 auto (4- and 8-byte storage)
     int tune_two_zc;      // planes per z-chunk of the two-step launch; 0 = chosen from the grid
+    int tune_two_bx;      // two-step tile width: 32 or 64 cells; 0 = automatic
     int num_cu;           // compute units of the state's device
     bool post_collision;  // fused: buf[cur] holds post-collision populations (pull step)
     // one-array state (PLB_MODE_FUSED_AA): buf[cur] only; aa_odd = after an
@@ -127,6 +128,8 @@ int timer_end(plb_state* s, TimerPair* tp);
 // state steps with the pull scheme and a kernel exists for its lattice,
 // storage and row length; the launch needs post-collision storage.
 bool fused_two_eligible(const plb_state* s);
+// The two-step tile width for the state (32 or 64 cells), or 0 where none applies.
+int fused_two_tile_x(const plb_state* s);
 int fused_launch_two(plb_state* s);
 
 // literal mode (literal.hip)

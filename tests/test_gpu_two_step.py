@@ -25,6 +25,7 @@ CASES = [
     (19, 64, 1, 5, None, False, 1),
     (27, 64, 11, 9, "z", True, 4),
     (27, 64, 6, 6, "y", False, 0),
+    (19, 96, 9, 7, "z", True, 3),
 ]
 
 
@@ -43,9 +44,10 @@ def bcs(walls, lid=(0.03, 0.0, 0.0)):
     return Boundary(type=t, vel=v), P.boundary(t, v)
 
 
-def two_step(st, on=True, zc=0):
+def two_step(st, on=True, zc=0, bx=0):
     st.set_tuning(capi.TUNE_TWO_STEP, 1 if on else 0)
     st.set_tuning(capi.TUNE_TWO_ZCHUNK, zc)
+    st.set_tuning(capi.TUNE_TWO_TILE_X, bx)
 
 
 @pytest.mark.parametrize("q,X,Y,Z,walls,solids,zc", CASES)
@@ -62,24 +64,31 @@ def test_two_step_matches_oracle(orc, q, X, Y, Z, walls, solids, zc, steps):
         o = orc.cor_state(variant, q, X, Y, Z)
         orc.cor_init(o, 1.0, (0.01, 0.0, 0.0), init)
         orc.cor_run(o, tau, F, ob, steps, mask)
-        with P.State(q, X, Y, Z, variant, P.FUSED) as st:
-            two_step(st, True, zc)
-            st.set_boundary(pb, mask)
-            st.initialize(1.0, (0.01, 0.0, 0.0), init, pb)
-            st.configure(tau, F, pb)
-            st.step(steps)
-            raw = st.download_raw()
-            rho, u = st.moments()
-        if variant == "f16s":
-            assert np.array_equal(raw.view(np.uint16)[fluid], o["f"].view(np.uint16)[fluid]), variant
-        else:
-            assert np.array_equal(raw[fluid], o["f"][fluid]), variant
         orho, ou = orc.cor_moments(o, ob, mask)
-        assert np.array_equal(rho, orho) and np.array_equal(u, ou), variant
+        # 64- and 32-cell tiles; per-population base pointers (the form for
+        # arrays beyond 4 GiB, forced by the 64-bit index setting)
+        for bx, wide in ((64, 0), (32, 0), (64, 1)):
+            if X % bx:
+                continue
+            with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+                two_step(st, True, zc, bx)
+                st.set_tuning(capi.TUNE_INDEX64, wide)
+                assert st.scheme() == "pull2", (variant, bx, wide)
+                st.set_boundary(pb, mask)
+                st.initialize(1.0, (0.01, 0.0, 0.0), init, pb)
+                st.configure(tau, F, pb)
+                st.step(steps)
+                raw = st.download_raw()
+                rho, u = st.moments()
+            if variant == "f16s":
+                assert np.array_equal(raw.view(np.uint16)[fluid], o["f"].view(np.uint16)[fluid]), (variant, bx, wide)
+            else:
+                assert np.array_equal(raw[fluid], o["f"][fluid]), (variant, bx, wide)
+            assert np.array_equal(rho, orho) and np.array_equal(u, ou), (variant, bx, wide)
 
 
-@pytest.mark.parametrize("variant", ["f64", "f32s", "f16s"])
-def test_two_step_equals_one_step_with_reads_and_graphs(orc, variant):
+@pytest.mark.parametrize("variant,bx", [("f64", 64), ("f32s", 64), ("f16s", 64), ("f32s", 32)])
+def test_two_step_equals_one_step_with_reads_and_graphs(orc, variant, bx):
     """Runs split by reads (each read gathers the pending streaming), graph
     replays of 10 steps (four two-step launches and two one-step launches, so
     that the graph keeps its buffer parity) and of 12 (six two-step launches),
@@ -89,7 +98,7 @@ def test_two_step_equals_one_step_with_reads_and_graphs(orc, variant):
     mask = (np.random.default_rng(4).random((Z, Y, X)) < 0.08).astype(np.uint8)
     fl = orc.cor_flags(q, X, Y, Z, ob, mask) == 0
     with P.State(q, X, Y, Z, variant, P.FUSED) as a, P.State(q, X, Y, Z, variant, P.FUSED) as b:
-        two_step(a, True, 4)
+        two_step(a, True, 4, bx)
         two_step(b, False)
         for st in (a, b):
             st.set_boundary(pb, mask)
@@ -115,17 +124,21 @@ def test_two_step_knobs():
             st.set_tuning(capi.TUNE_TWO_STEP, 2)
         with pytest.raises(P.InvalidArgument):
             st.set_tuning(capi.TUNE_TWO_ZCHUNK, -1)
+        with pytest.raises(P.InvalidArgument):
+            st.set_tuning(capi.TUNE_TWO_TILE_X, 48)
         st.set_tuning(capi.TUNE_TWO_STEP, -1)
         st.set_tuning(capi.TUNE_TWO_ZCHUNK, 0)
         assert st.scheme() == "pull2" and st.steps_per_launch() == 2
         st.set_tuning(capi.TUNE_TWO_STEP, 0)
         assert st.scheme() == "pull"
-    # automatic: 4-byte storage only; forced on where a kernel exists
+    # automatic: 4- and 8-byte storage; forced on where a kernel exists
     with P.State(19, 64, 8, 8, "f16s", P.FUSED) as st:
         assert st.scheme() == "pull"
         st.set_tuning(capi.TUNE_TWO_STEP, 1)
         assert st.scheme() == "pull2"
-    for q, x, v in ((19, 96, "f32s"), (9, 64, "f32s"), (27, 64, "f64")):
+    with P.State(19, 64, 8, 8, "f64", P.FUSED) as st:
+        assert st.scheme() == "pull2"
+    for q, x, v in ((19, 80, "f32s"), (9, 64, "f32s"), (27, 64, "f64")):
         with P.State(q, x, 1 if q == 9 else 8, 8, v, P.FUSED) as st:
             st.set_tuning(capi.TUNE_TWO_STEP, 1)
             assert st.scheme() == "pull", (q, x, v)
@@ -134,8 +147,9 @@ def test_two_step_knobs():
         assert st.scheme() == "push"
 
 
-@pytest.mark.parametrize("variant,solids", [("f32s", True), ("f32s", False), ("f16s", True), ("f64", True)])
-def test_two_step_full_size(variant, solids):
+@pytest.mark.parametrize("variant,solids,bx", [("f32s", True, 64), ("f32s", False, 64), ("f16s", True, 64),
+                                               ("f64", True, 64), ("f32s", True, 32)])
+def test_two_step_full_size(variant, solids, bx):
     """The headline lattice (D3Q19 256^3 channel, 10% solid voxels): two-step
     launches equal one-step launches bit for bit after 20 steps (graph replays,
     the launch geometry of the benchmark: eight XCD stripes, two z-chunks)."""
@@ -146,7 +160,7 @@ def test_two_step_full_size(variant, solids):
     out = []
     for on in (True, False):
         with P.State(19, N, N, N, variant, P.FUSED) as st:
-            two_step(st, on)
+            two_step(st, on, 0, bx)
             st.set_boundary(bc, mask)
             st.initialize(1.0, (0.0, 0.0, 0.0), (P.INIT_PERTURBED, 3, 1e-3), bc)
             st.configure(0.55, (5e-7, 0.0, 0.0), bc)
