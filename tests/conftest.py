@@ -10,7 +10,7 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
-    config.addinivalue_line("markers", "slow: longer parity runs at BASELINE sizes")
+    config.addinivalue_line("markers", "slow: longer parity runs at the full configuration sizes")
 
 
 @pytest.fixture(scope="session")
