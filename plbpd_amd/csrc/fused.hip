@@ -1,4 +1,5 @@
-// Fused mode: one kernel per time step does the work of CollisionSRT,
+// Fused mode: one kernel per time step (or per two steps: fused_two.hpp)
+// does the work of CollisionSRT,
 // Streaming, Boundary::apply_noslip / apply_periodicity and the moments that
 // the next collision needs (CollisionSRT.cpp:153-176, Streaming.cpp:63-74,
 // Boundary.cpp:189-281, CalcMoments.hpp:22-37), without the stale seam reads
@@ -6,9 +7,10 @@
 //
 // Layout in HBM (structure of arrays, no halo):
 //   f[k * pstride + c],  c = (z*Y + y)*X + x,  x fastest.
-// Two schemes give the same numbers.  The pull step (the default; k_pull here
-// and the row kernel of fused_row.hpp) gathers each population from x - c_k
-// and writes the cell's own slots, so every store is a whole aligned line.
+// Two schemes give the same numbers.  The pull step (the default; k_pull here,
+// the row kernel of fused_row.hpp, and two steps per launch in fused_two.hpp)
+// gathers each population from x - c_k and writes the cell's own slots, so
+// every store is a whole aligned line.
 // The push step (k_push) reads the cell's own slots and writes each
 // population at x + c_k.  One thread per cell in both cell kernels: a block
 // covers TX x-consecutive cells of TY rows, so a wavefront touches 64
