@@ -147,6 +147,11 @@ extern "C" {
 
 int plb_set_tuning(plb_state* s, int key, int value) {
     if (!s) return fail(PLB_ERR_INVALID_ARGUMENT, "null state");
+    if (s->fused_deferred) {  // a step the staged calls deferred runs with the old settings
+        PLB_HIP_TRY(hipSetDevice(s->device));
+        const int rc = fused_flush_deferred(s);
+        if (rc) return rc;
+    }
     switch (key) {
         case PLB_TUNE_BLOCK_X:
             if (value != 0 && value != 64 && value != 128 && value != 256)

@@ -152,9 +152,11 @@ LitOp lit_op(LitOpKind kind, const plb_boundary* bc = nullptr, double tau = 0.0,
 }
 
 int enter(plb_state* s) {
-    const int rc = enter_keep(s);
+    int rc = enter_keep(s);
     if (rc) return rc;
-    return lit_flush(s);
+    rc = lit_flush(s);
+    if (rc) return rc;
+    return fused_flush_deferred(s);
 }
 
 bool same_bc(const plb_boundary& a, const plb_boundary& b) {
@@ -163,11 +165,21 @@ bool same_bc(const plb_boundary& a, const plb_boundary& b) {
 
 int ensure_fused_bc(plb_state* s, const plb_boundary* bc, std::vector<uint8_t>* mask) {
     if (s->bc_set && same_bc(s->bc, *bc)) return PLB_OK;
+    // a deferred step runs with the boundary it was taken with
+    const int rc = fused_flush_deferred(s);
+    if (rc) return rc;
     drop_graph(s);
     return fused_build_flags(s, bc, mask && !mask->empty() ? mask->data() : nullptr);
 }
 
 }  // namespace
+
+int fused_flush_deferred(plb_state* s) {
+    if (!s->fused_deferred) return PLB_OK;
+    s->fused_deferred = false;
+    return timed(s, PLB_STAGE_FUSED, [&] { return fused_launch(s, true); });
+}
+
 }  // namespace plb
 
 using namespace plb;
@@ -268,6 +280,7 @@ int plb_state_create(const plb_state_desc* d, plb_state** out) {
     s->force[0] = s->force[1] = s->force[2] = 0.0;
     s->configured = false;
     s->collide_pending = false;
+    s->fused_deferred = false;
     s->lit_ntmp_folded = false;
     s->moments_valid = false;
     s->scratch = nullptr;
@@ -468,7 +481,11 @@ int plb_collide_srt(plb_state* s, double tau, const double force[3]) {
         return PLB_OK;
     }
     if (!force) return fail(PLB_ERR_INVALID_ARGUMENT, "null force");
-    if (s->tau != tau || std::memcmp(s->force, force, sizeof(s->force)) != 0) drop_graph(s);
+    if (s->tau != tau || std::memcmp(s->force, force, sizeof(s->force)) != 0) {
+        // a deferred step runs with the relaxation time and force it was taken with
+        if ((rc = fused_flush_deferred(s))) return rc;
+        drop_graph(s);
+    }
     s->tau = tau;
     for (int i = 0; i < 3; ++i) s->force[i] = force[i];
     s->collide_pending = true;
@@ -508,11 +525,26 @@ int plb_stream(plb_state* s, const char* stream_type) {
         return fail(PLB_ERR_LOGIC, "stream type pull with the push scheme selected (PLB_TUNE_SCHEME)");
     const bool collide = s->collide_pending;
     s->collide_pending = false;
+    // Whole steps of the staged calls pair up into two-step launches: the
+    // first is recorded, the second launches both.  Every call that observes
+    // or changes the state runs a recorded step first (enter(), and
+    // ensure_fused_bc and plb_collide_srt when they change a parameter).
+    if (collide && s->post_collision && fused_two_eligible(s)) {
+        s->moments_valid = false;
+        if (!s->fused_deferred) {
+            s->fused_deferred = true;
+            return PLB_OK;
+        }
+        s->fused_deferred = false;
+        return timed(s, PLB_STAGE_FUSED, [&] { return fused_launch_two(s); });
+    }
+    if ((rc = fused_flush_deferred(s))) return rc;
     return timed(s, PLB_STAGE_FUSED, [&] { return fused_launch(s, collide); });
 }
 
 int plb_apply_noslip(plb_state* s, const plb_boundary* bc) {
-    int rc = enter(s);
+    // fused: a check of the folded boundary, which keeps a deferred step
+    int rc = s && s->mode == PLB_MODE_LITERAL ? enter(s) : enter_keep(s);
     if (rc) return rc;
     if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
     if (s->mode == PLB_MODE_LITERAL) return timed(s, PLB_STAGE_NOSLIP, [&] { return lit_noslip(s, bc); });
@@ -540,7 +572,7 @@ int plb_apply_noslip_noalloc(plb_state* s, const plb_boundary* bc) {
 }
 
 int plb_apply_periodicity(plb_state* s, const plb_boundary* bc) {
-    int rc = enter(s);
+    int rc = s && s->mode == PLB_MODE_LITERAL ? enter(s) : enter_keep(s);
     if (rc) return rc;
     if (!bc) return fail(PLB_ERR_INVALID_ARGUMENT, "null boundary");
     if (s->mode == PLB_MODE_LITERAL) return timed(s, PLB_STAGE_PERIODIC, [&] { return lit_periodic(s, bc); });
@@ -548,7 +580,7 @@ int plb_apply_periodicity(plb_state* s, const plb_boundary* bc) {
 }
 
 int plb_calc_moments(plb_state* s) {
-    int rc = enter(s);
+    int rc = s && s->mode == PLB_MODE_LITERAL ? enter(s) : enter_keep(s);
     if (rc) return rc;
     if (s->mode == PLB_MODE_LITERAL) {
         if (!s->tune_lit_fuse) return timed(s, PLB_STAGE_MOMENTS, [&] { return lit_moments(s); });

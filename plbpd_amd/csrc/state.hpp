@@ -78,6 +78,10 @@ struct plb_state {
     double force[3];
     bool configured;
     bool collide_pending;      // fused: the next fused launch collides first
+    // fused: one collide+stream step that plb_stream recorded without
+    // launching it; the next plb_stream runs it with its own step as one
+    // two-step launch, and any call that needs it done runs it first
+    bool fused_deferred;
     // literal: calls deferred so that a following push can fold them into
     // its kernel (at most moments, reset velocity, reset density, collide, in
     // that order); any other call runs them first, in order
@@ -128,6 +132,8 @@ int timer_end(plb_state* s, TimerPair* tp);
 // state steps with the pull scheme and a kernel exists for its lattice,
 // storage and row length; the launch needs post-collision storage.
 bool fused_two_eligible(const plb_state* s);
+// Runs the step plb_stream deferred, if any (capi.hip).
+int fused_flush_deferred(plb_state* s);
 // The two-step tile width for the state (32 or 64 cells), or 0 where none applies.
 int fused_two_tile_x(const plb_state* s);
 int fused_launch_two(plb_state* s);

@@ -171,3 +171,78 @@ def test_two_step_full_size(variant, solids, bx):
     fluid[-1] = False
     assert np.array_equal(out[0][0][fluid], out[1][0][fluid])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("variant", ["f32s", "f64"])
+def test_staged_calls_pair_into_two_step_launches(orc, variant):
+    """The reference's staged calls (collide, stream, no-slip, periodicity,
+    moments, reset) on a two-step state: a whole step is recorded by the
+    stream call and launched together with the next one.  Reads after odd
+    numbers of steps, a change of relaxation time and force, a boundary change
+    (a new lid velocity), a stream without a collision and a tuning change all
+    run the recorded step first; the populations equal those of one-step
+    launches bit for bit, and plb_step's."""
+    q, X, Y, Z = 19, 64, 12, 9
+    ob, pb = bcs("z", lid=(0.02, 0.0, 0.0))
+    _, pb2 = bcs("z", lid=(-0.01, 0.015, 0.0))
+    mask = (np.random.default_rng(23).random((Z, Y, X)) < 0.08).astype(np.uint8)
+    fl = orc.cor_flags(q, X, Y, Z, ob, mask) == 0
+    F1, F2 = (1e-5, 0.0, 0.0), (2e-5, -1e-6, 0.0)
+    with P.State(q, X, Y, Z, variant, P.FUSED) as a, P.State(q, X, Y, Z, variant, P.FUSED) as b, \
+            P.State(q, X, Y, Z, variant, P.FUSED) as c:
+        two_step(a, True)
+        two_step(b, False)
+        two_step(c, True)
+        for st in (a, b, c):
+            st.set_boundary(pb, mask)
+            st.initialize(1.0, (0, 0, 0), (P.INIT_PERTURBED, 4, 1e-3), pb)
+
+        def staged(st, n, tau, F, bc, collide=True):
+            for _ in range(n):
+                if collide:
+                    st.collide(tau, F)
+                st.stream("push")
+                st.apply_noslip(bc)
+                st.apply_periodicity(bc)
+                st.calc_moments()
+                st.reset(bc)
+
+        def same(tag):
+            assert np.array_equal(a.download_raw()[fl], b.download_raw()[fl]), tag
+            assert np.array_equal(c.download_raw()[fl], b.download_raw()[fl]), tag
+
+        for st in (a, b):
+            staged(st, 7, 0.6, F1, pb)
+        c.configure(0.6, F1, pb)
+        c.step(7)
+        same("7 steps")
+        for st in (a, b):
+            staged(st, 5, 0.6, F1, pb)
+            staged(st, 4, 0.7, F2, pb)  # new relaxation time and force after an odd count
+        c.step(5)
+        c.configure(0.7, F2, pb)
+        c.step(4)
+        same("tau change")
+        for st in (a, b):
+            staged(st, 3, 0.7, F2, pb)
+            # new lid velocity after an odd count: the staged calls hand the
+            # boundary over after the stream, so the first of these steps
+            # still streams with the old one
+            staged(st, 3, 0.7, F2, pb2)
+        c.step(4)
+        c.configure(0.7, F2, pb2)
+        c.step(2)
+        same("boundary change")
+        for st in (a, b):
+            staged(st, 1, 0.7, F2, pb2)
+            staged(st, 1, 0.7, F2, pb2, collide=False)  # a stream without a collision
+            staged(st, 2, 0.7, F2, pb2)
+        assert np.array_equal(a.download_raw()[fl], b.download_raw()[fl]), "stream only"
+        staged(a, 3, 0.7, F2, pb2)
+        a.set_tuning(capi.TUNE_TWO_STEP, 0)  # the recorded step runs before the change
+        staged(a, 2, 0.7, F2, pb2)
+        staged(b, 5, 0.7, F2, pb2)
+        ra, ua = a.moments()
+        rb, ub = b.moments()
+        assert np.array_equal(ra, rb) and np.array_equal(ua, ub)
+        assert np.array_equal(a.global_moments(), b.global_moments())
