@@ -246,3 +246,28 @@ def test_staged_calls_pair_into_two_step_launches(orc, variant):
         rb, ub = b.moments()
         assert np.array_equal(ra, rb) and np.array_equal(ua, ub)
         assert np.array_equal(a.global_moments(), b.global_moments())
+
+
+@pytest.mark.parametrize("q,X,Y,Z,variant", [(19, 128, 100, 77, "f32s"), (19, 192, 41, 33, "f64"),
+                                             (27, 384, 40, 29, "f32"), (19, 96, 57, 40, "f32s")])
+def test_two_step_odd_shapes(q, X, Y, Z, variant):
+    """Launch geometries the small oracle cases do not reach: y-tile counts
+    that are not a multiple of 8 (no XCD stripes), ragged last tiles and
+    z-chunks, 384-cell rows, 32-cell tiles (x = 96); against one-step
+    launches, bit for bit, with solids and a moving lid."""
+    ob, pb = bcs("z", lid=(0.02, 0.0, 0.01))
+    mask = (np.random.default_rng(31).random((Z, Y, X)) < 0.1).astype(np.uint8)
+    out = []
+    for on in (True, False):
+        with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+            two_step(st, on)
+            assert st.scheme() == ("pull2" if on else "pull")
+            st.set_boundary(pb, mask)
+            st.initialize(1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 8, 1e-3), pb)
+            st.configure(0.57, (1e-5, 0.0, 0.0), pb)
+            st.step(9)
+            out.append(st.download_raw())
+    fluid = mask == 0
+    fluid[0] = False
+    fluid[-1] = False
+    assert np.array_equal(out[0][fluid], out[1][fluid])
