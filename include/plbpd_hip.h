@@ -315,9 +315,13 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
                                 /* (one read and one write of the populations per two */
                                 /* steps, the same values bit for bit) wherever a     */
                                 /* kernel exists: D3Q19, and D3Q27 with 2- or 4-byte  */
-                                /* storage, x a multiple of 64, 32-bit offsets; 0     */
-                                /* one step per launch; -1 (default) automatic: on    */
-                                /* for 4- and 8-byte storage                           */
+                                /* storage; x a multiple of 32 (of 64 for arrays over */
+                                /* 4 GiB); one population array below 4 GiB; 0 one   */
+                                /* step per launch; -1 (default) automatic: on for    */
+                                /* 4- and 8-byte storage.  The staged calls pair      */
+                                /* whole steps (plb_stream records one step and the   */
+                                /* next plb_stream launches both; any call that reads */
+                                /* or changes the state runs a recorded step first)   */
 #define PLB_TUNE_TWO_ZCHUNK 10  /* planes per z-chunk of a two-step launch (0: chosen */
                                 /* from the grid and the compute-unit count)          */
 #define PLB_TUNE_TWO_TILE_X 11  /* two-step tile width: 64 cells (one workgroup per   */
