@@ -51,8 +51,14 @@ __device__ inline void gather_walls(typename Prec<V>::S (&fv)[Q], const typename
                                     uint32_t c, uint32_t ps, const Seams<uint32_t>& s) {
     if constexpr (K < Q) {
         const bool wall = pull_wall<Q, K>(s);
-        fv[K] = *pop_elem<W>(A, wall ? (uint32_t)rev_k<Q>(K) : (uint32_t)K, ps,
-                             wall ? c : c + pull_offset<Q, K>(s));
+        if constexpr (W) {
+            fv[K] = *pop_elem<W>(A, wall ? (uint32_t)rev_k<Q>(K) : (uint32_t)K, ps,
+                                 wall ? c : c + pull_offset<Q, K>(s));
+        } else {
+            const uint32_t moved = (uint32_t)K * ps + (c + pull_offset<Q, K>(s));
+            const uint32_t own = (uint32_t)rev_k<Q>(K) * ps + c;
+            fv[K] = *elem(A, wall ? own : moved);
+        }
         gather_walls<Q, K + 1, V, W>(fv, A, c, ps, s);
     }
 }
@@ -66,7 +72,16 @@ __device__ inline void gather_mask(typename Prec<V>::S (&fv)[Q], const typename 
     if constexpr (K < Q) {
         constexpr uint32_t R = (uint32_t)rev_k<Q>(K);
         const bool solid = ((m >> R) & 1u) != 0;
-        fv[K] = *pop_elem<W>(A, solid ? R : (uint32_t)K, ps, solid ? c : c + pull_offset<Q, K>(s));
+        if constexpr (W) {
+            fv[K] = *pop_elem<W>(A, solid ? R : (uint32_t)K, ps, solid ? c : c + pull_offset<Q, K>(s));
+        } else {
+            // select between two offsets whose population parts are uniform
+            // (a selected population number times pstride would be a per-lane
+            // multiply)
+            const uint32_t moved = (uint32_t)K * ps + (c + pull_offset<Q, K>(s));
+            const uint32_t own = R * ps + c;
+            fv[K] = *elem(A, solid ? own : moved);
+        }
         gather_mask<Q, K + 1, V, W>(fv, A, c, ps, m, s);
     }
 }
