@@ -915,9 +915,9 @@ int fused_two_tile_x(const plb_state* s) {
 bool fused_two_eligible(const plb_state* s) {
     if (s->tune_two == 0 || s->aa || !fused_uses_pull(s) || s->mode != PLB_MODE_FUSED) return false;
     // automatic: 4- and 8-byte storage, where two steps per pass measured
-    // 1.28x (D3Q19 256^3 FP32-shifted, 10% solids), 1.56x (walls only), 1.07x
-    // and 1.39x (FP64) and 1.35x (D3Q27 FP32-shifted, walls) faster on MI355X;
-    // 2-byte storage ran slower (1.34x with solids: its one-step kernel is
+    // 1.40x (D3Q19 256^3 FP32-shifted, 10% solids), 1.57x (walls only), 1.26x
+    // and 1.41x (FP64) and 1.35x (D3Q27 FP32-shifted, walls) faster on MI355X;
+    // 2-byte storage ran 1.23x slower with solids (its one-step kernel is
     // closer to the arithmetic limit)
     if (s->tune_two < 0 && s->sbytes == 2) return false;
     if ((s->q != 19 && s->q != 27) || fused_two_tile_x(s) == 0) return false;

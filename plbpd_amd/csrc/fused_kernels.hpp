@@ -331,14 +331,29 @@ __host__ __device__ constexpr int two_cells(int bx, int by) { return (bx + 2) * 
 __host__ __device__ constexpr int two_threads(int bx, int by) { return (two_cells(bx, by) + 63) / 64 * 64; }
 // Workgroups per compute unit the LDS budget of a tile width allows.
 __host__ __device__ constexpr int two_wgs(int bx) { return bx == 32 ? 2 : 1; }
+// Whether the two-step kernel issues the step t gathers of the c_z = +1
+// directions an iteration earlier than the others (fused_two.hpp): D3Q19
+// with 4-byte storage, and with 8-byte storage around interior solids (nb).
+// The values held for the extra iteration fit there without spills (at 168
+// VGPRs with 4-byte storage, 256 with 8-byte); D3Q27 (9 such directions) and
+// 2-byte storage (13-row tiles at 128 VGPRs) spilled, and 8-byte storage
+// without interior solids measured 1% slower.
+__host__ __device__ constexpr bool two_split(int q, int sbytes, bool nb) {
+    return q == 19 && (sbytes == 4 || (sbytes == 8 && nb));
+}
+
 // Rows per tile for a lattice, a storage width and a tile width: the largest
 // BY whose ring ((bx + 2) x (BY + 2) cells x (zring_planes(q) population
 // planes + two 4-byte mask planes)) fits the LDS share of a workgroup
 // (160 KiB per gfx950 compute unit, split between two_wgs(bx) workgroups),
-// within 1024 threads; 0 where no BY >= 3 fits.
+// within 1024 threads; 0 where no BY >= 3 fits.  The split kernel with
+// 4-byte storage takes at most 8 rows: its 168 VGPRs allow 12 wavefronts
+// per compute unit (BY <= 9 at 64-cell tiles), and 8-row tiles cut rows of
+// a multiple of 64 cells into a tile count divisible by the 8 XCDs.
 __host__ __device__ constexpr int two_rows(int q, int sbytes, int bx) {
     int best = 0;
-    for (int by = 1; by <= 30; ++by)
+    const int cap = two_split(q, sbytes, true) && sbytes == 4 ? 8 : 30;
+    for (int by = 1; by <= cap; ++by)
         if (two_cells(bx, by) * (zring_planes(q) * sbytes + 8) <= 160 * 1024 / two_wgs(bx) &&
             two_cells(bx, by) <= 1024)
             best = by;

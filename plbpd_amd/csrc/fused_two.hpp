@@ -18,11 +18,14 @@
 // storage type on their way into LDS, exactly as a single step stores them,
 // so the result equals two single steps bit for bit.  The loads of plane
 // zi + 1 are issued before the step t + 2 work of plane zi - 1, whose
-// arithmetic hides their latency.  Each lattice instantiates these templates
-// in its own file (fused_two_q19.hip, fused_two_q27.hip).
+// arithmetic hides their latency; where two_split() holds, those of the
+// c_z = +1 directions a plane earlier still, so that the bounce-back reads of
+// a cache line fall in the iteration of its pulls.  Each lattice instantiates
+// these templates in its own file (fused_two_q19.hip, fused_two_q27.hip).
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 
 #include "fused_kernels.hpp"
 
@@ -43,13 +46,24 @@ __device__ inline S* pop_elem(S* A, uint32_t k, uint32_t ps, uint32_t i) {
         return elem(A, k * ps + i);
 }
 
-// Step t loads without interior solids: population k from x - c_k, or the
-// cell's own slot rev(k) where x - c_k is a wall layer (known from the
-// coordinates, so no load waits on a flag).
-template <int Q, int K, int V, bool W>
+// Direction groups of the step t gathers: kAll, or, where the c_z = +1
+// directions are issued a plane ahead of the others, kUp (c_z = +1) and
+// kFlat (c_z = 0 and -1).
+enum : int { kAll = 0, kUp = 1, kFlat = 2 };
+template <int Q, int K, int G>
+__device__ constexpr bool in_group() {
+    return G == kAll || (Lat<Q>::cz(K) > 0) == (G == kUp);
+}
+
+// Step t loads without interior solids, for the directions of group G:
+// population k from x - c_k, or the cell's own slot rev(k) where x - c_k is
+// a wall layer (known from the coordinates, so no load waits on a flag).
+template <int Q, int K, int V, bool W, int G>
 __device__ inline void gather_walls(typename Prec<V>::S (&fv)[Q], const typename Prec<V>::S* __restrict__ A,
                                     uint32_t c, uint32_t ps, const Seams<uint32_t>& s) {
-    if constexpr (K < Q) {
+    if constexpr (K < Q && !in_group<Q, K, G>()) {
+        gather_walls<Q, K + 1, V, W, G>(fv, A, c, ps, s);
+    } else if constexpr (K < Q) {
         const bool wall = pull_wall<Q, K>(s);
         if constexpr (W) {
             fv[K] = *pop_elem<W>(A, wall ? (uint32_t)rev_k<Q>(K) : (uint32_t)K, ps,
@@ -59,17 +73,19 @@ __device__ inline void gather_walls(typename Prec<V>::S (&fv)[Q], const typename
             const uint32_t own = (uint32_t)rev_k<Q>(K) * ps + c;
             fv[K] = *elem(A, wall ? own : moved);
         }
-        gather_walls<Q, K + 1, V, W>(fv, A, c, ps, s);
+        gather_walls<Q, K + 1, V, W, G>(fv, A, c, ps, s);
     }
 }
 
-// Step t loads around interior solids: population k from x - c_k, or the
-// cell's own slot rev(k) where the neighbour mask m says x - c_k is not fluid
-// (one load per population; the mask is loaded an iteration ahead).
-template <int Q, int K, int V, bool W>
+// Step t loads around interior solids, for the directions of group G:
+// population k from x - c_k, or the cell's own slot rev(k) where the
+// neighbour mask m says x - c_k is not fluid (one load per population).
+template <int Q, int K, int V, bool W, int G>
 __device__ inline void gather_mask(typename Prec<V>::S (&fv)[Q], const typename Prec<V>::S* __restrict__ A,
                                    uint32_t c, uint32_t ps, uint32_t m, const Seams<uint32_t>& s) {
-    if constexpr (K < Q) {
+    if constexpr (K < Q && !in_group<Q, K, G>()) {
+        gather_mask<Q, K + 1, V, W, G>(fv, A, c, ps, m, s);
+    } else if constexpr (K < Q) {
         constexpr uint32_t R = (uint32_t)rev_k<Q>(K);
         const bool solid = ((m >> R) & 1u) != 0;
         if constexpr (W) {
@@ -82,7 +98,7 @@ __device__ inline void gather_mask(typename Prec<V>::S (&fv)[Q], const typename 
             const uint32_t own = R * ps + c;
             fv[K] = *elem(A, solid ? own : moved);
         }
-        gather_mask<Q, K + 1, V, W>(fv, A, c, ps, m, s);
+        gather_mask<Q, K + 1, V, W, G>(fv, A, c, ps, m, s);
     }
 }
 
@@ -155,7 +171,7 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int Q, int V, int NB, bool MOV, int BX, int BY, bool W>
+template <int Q, int V, int NB, bool MOV, int BX, int BY, bool W, bool SPLIT>
 __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(const typename Prec<V>::S* __restrict__ A,
                                                            typename Prec<V>::S* __restrict__ B,
                                                            const uint8_t* __restrict__ flags,
@@ -198,12 +214,19 @@ __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(cons
     const uint32_t px = x0 + ox, py = y0 + oy;
     const I li = (oy + 1u) * RX + ox + 1u;
 
-    // Loads of step t for the plane of iteration n, issued one iteration
-    // ahead.  Around interior solids the gather addresses depend on the
-    // neighbour mask, which is loaded at the start of the iteration before,
-    // so that it has arrived when the gather is issued.
-    S fv[Q];
-    // neighbour mask (NB) or flag of this thread's step t + 1 cell in plane n
+    // Loads of step t, issued an iteration ahead of the plane that uses them.
+    // SPLIT issues the directions with c_z = +1 two iterations ahead instead,
+    // so that the bounce-back read of a cell's own slot rev(k) falls in the
+    // same iteration as the pulls from the same cache line: a line of a
+    // c_z = +1 population in plane p is pulled by the cells of plane p + 1
+    // and read back by those of plane p for their c_z = -1 directions, and a
+    // line of a c_z = -1 population is pulled by the cells of plane p - 1 and
+    // read back, for their c_z = +1 directions, by those of plane p.  (The
+    // c_z = +1 values then stay in registers for an extra iteration.)
+    // Around interior solids the gather addresses depend on the neighbour
+    // mask, loaded at the start of the iteration before the gather.
+    //
+    // Neighbour mask (NB) or flag of this thread's step t + 1 cell in plane n.
     // Every thread loads, also past the grown tile (its wrapped coordinates
     // stay in the lattice; the values are not used): loads under a branch
     // made the compiler wait for loads in flight where the branches merge.
@@ -211,77 +234,122 @@ __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(cons
         const I c = ((z0 + n + Z - 1u) % Z) * XY + gy * X + gx;
         return NB ? nbmask[c] : (uint32_t)flags[c];
     };
-    auto fetch = [&](uint32_t n, uint32_t m) {
+    // the loads of plane n's directions of group G into fv
+    auto fetch = [&](auto group, uint32_t n, S (&fv)[Q], uint32_t m) {
+        constexpr int G = decltype(group)::value;
         const uint32_t zi = (z0 + n + Z - 1u) % Z;
         const I c = zi * XY + gy * X + gx;
         const Seams<I> sm = seams<I>(gx, gy, zi, a);
         if (NB)
-            gather_mask<Q, 0, V, W>(fv, A, c, ps, m, sm);
+            gather_mask<Q, 0, V, W, G>(fv, A, c, ps, m, sm);
         else
-            gather_walls<Q, 0, V, W>(fv, A, c, ps, sm);
+            gather_walls<Q, 0, V, W, G>(fv, A, c, ps, sm);
     };
+    using All = std::integral_constant<int, kAll>;
+    using Up = std::integral_constant<int, kUp>;
+    using Flat = std::integral_constant<int, kFlat>;
 
-    // One plane: the step t + 1 cell of plane n from the loads in fv and the
-    // mask mcur; the mask of plane n + 1 into mnext and its loads into fv;
-    // the step t + 2 cell of plane n - 1.  The loop runs it twice per trip
-    // with the two mask registers swapped, so that no register that a load
-    // in flight writes is copied (a copy would wait for the load).
-    auto plane = [&](uint32_t n, uint32_t& mcur, uint32_t& mnext) {
-        // (plane numbers wrap modulo Z, so the loads past the chunk's last
-        // plane stay in bounds; their values are not used)
-        mnext = load_mask(n + 1u);
-        // ring slots of plane n (relative plane numbers: only n mod 3 and mod 2 matter)
-        const uint32_t a3 = n % 3u, a2 = n & 1u;
-        // step t + 1 at plane zi into the ring
-        if (in_i) {
-            mring[a2 * NI + tid] = mcur;
-            const bool self_solid = NB ? (mcur & 1u) != 0 : mcur != 0;
+    // Step t + 2 of the tile's cell at plane n - 2 of the chunk (iteration n):
+    // from the ring planes of iterations n - 2, n - 1 and n, with the mask of
+    // its step t + 1 cell (stored in mring by iteration n - 1).
+    auto step2 = [&](uint32_t n) {
+        const uint32_t zo = z0 + n - 2u;
+        const I c = zo * XY + py * X + px;
+        const uint32_t pa = (n + 1u) % 3u, oa = (n + 2u) % 3u, b0 = (n + 1u) & 1u, c1 = n & 1u;
+        const uint32_t mo = mring[b0 * NI + li];
+        const bool self_solid = NB ? (mo & 1u) != 0 : mo != 0;
+        if (!__all(self_solid)) {
+            const Seams<I> sm = seams<I>(px, py, zo, a);
+            T f[Q];
+            ring_gather<Q, 0, V, NB, NI, RX>(f, ring, li, pa, oa, b0, c1, mo, sm);
             if (!self_solid) {
-                T f[Q];
-#pragma unroll
-                for (int k = 0; k < Q; ++k) f[k] = P::load_raw(fv[k]);
                 const T rho = bgk<Q, P>(f, a, true);
-                if (MOV && a.bc.moving) {
-                    const uint32_t zi = (z0 + n + Z - 1u) % Z;
-                    const I c = zi * XY + gy * X + gx;
-                    wall_terms<Q, 1, P>(f, rho, flags, c, seams<I>(gx, gy, zi, a), a.bc);
-                }
-                ring_store<Q, 0, V, NI>(f, ring, tid, a3, a2);
+                if (MOV && a.bc.moving) wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
             }
+            if (a.nt_store)
+                store_own<Q, V, true, W>(f, B, c, ps, self_solid);
+            else
+                store_own<Q, V, false, W>(f, B, c, ps, self_solid);
         }
-        fetch(n + 1u, mnext);
-        lds_barrier();
-        // step t + 2 at plane zo = zi - 1 from the ring
-        if (n >= 2u && in_o) {
-            const uint32_t zo = z0 + n - 2u;
-            const I c = zo * XY + py * X + px;
-            // planes zo - 1, zo, zo + 1 are the planes of iterations n - 2, n - 1, n
-            const uint32_t pa = (n + 1u) % 3u, oa = (n + 2u) % 3u, b0 = (n + 1u) & 1u, c1 = a2;
-            // this cell was a step t + 1 cell of the previous plane
-            const uint32_t mo = mring[b0 * NI + li];
-            const bool self_solid = NB ? (mo & 1u) != 0 : mo != 0;
-            if (!__all(self_solid)) {
-                const Seams<I> sm = seams<I>(px, py, zo, a);
-                T f[Q];
-                ring_gather<Q, 0, V, NB, NI, RX>(f, ring, li, pa, oa, b0, c1, mo, sm);
-                if (!self_solid) {
-                    const T rho = bgk<Q, P>(f, a, true);
-                    if (MOV && a.bc.moving) wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
-                }
-                if (a.nt_store)
-                    store_own<Q, V, true, W>(f, B, c, ps, self_solid);
-                else
-                    store_own<Q, V, false, W>(f, B, c, ps, self_solid);
-            }
+    };
+    // Step t + 1 of this thread's cell at plane n from the loads in fv, into
+    // the ring slots a3 (c_z = +1 group) and a2 (the others).
+    auto step1 = [&](uint32_t n, const S (&fv)[Q], uint32_t a3, uint32_t a2) {
+        T f[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) f[k] = P::load_raw(fv[k]);
+        const T rho = bgk<Q, P>(f, a, true);
+        if (MOV && a.bc.moving) {
+            const uint32_t zi = (z0 + n + Z - 1u) % Z;
+            const I c = zi * XY + gy * X + gx;
+            wall_terms<Q, 1, P>(f, rho, flags, c, seams<I>(gx, gy, zi, a), a.bc);
         }
-        lds_barrier();
+        ring_store<Q, 0, V, NI>(f, ring, tid, a3, a2);
     };
 
-    uint32_t m0 = load_mask(0), m1 = 0;
-    fetch(0, m0);
-    for (uint32_t n = 0; n < nplanes; n += 2u) {
-        plane(n, m0, m1);
-        if (n + 1u < nplanes) plane(n + 1u, m1, m0);
+    if constexpr (!SPLIT) {
+        // One plane n: the step t + 1 cell of plane n from the loads in fv
+        // and the mask mcur; the mask of plane n + 1 into mnext and its loads
+        // into fv; the step t + 2 cell of plane n - 1.  The loop runs it twice
+        // per trip with the two mask registers swapped, so that no register
+        // that a load in flight writes is copied (a copy would wait for the
+        // load).  (Written with one captured register set: the same body with
+        // the set passed by reference compiled to longer waits.)
+        S fv[Q];
+        auto plane = [&](uint32_t n, uint32_t& mcur, uint32_t& mnext) {
+            // (plane numbers wrap modulo Z, so the loads past the chunk's
+            // last plane stay in bounds; their values are not used)
+            mnext = load_mask(n + 1u);
+            // ring slots of plane n (relative plane numbers: only n mod 3 and mod 2 matter)
+            const uint32_t a3 = n % 3u, a2 = n & 1u;
+            if (in_i) {
+                // the step t + 2 cells read this mask in the next iteration
+                mring[a2 * NI + tid] = mcur;
+                const bool self_solid = NB ? (mcur & 1u) != 0 : mcur != 0;
+                if (!self_solid) step1(n, fv, a3, a2);
+            }
+            fetch(All{}, n + 1u, fv, mnext);
+            lds_barrier();
+            if (n >= 2u && in_o) step2(n);
+            lds_barrier();
+        };
+        uint32_t m0 = load_mask(0), m1 = 0;
+        fetch(All{}, 0, fv, m0);
+        for (uint32_t n = 0; n < nplanes; n += 2u) {
+            plane(n, m0, m1);
+            if (n + 1u < nplanes) plane(n + 1u, m1, m0);
+        }
+    } else {
+        // One plane n: the step t + 1 cell of plane n from fc and the mask mc
+        // of plane n; the c_z = 0 and -1 directions of plane n + 1 into fn
+        // (its c_z = +1 directions are in flight) with the mask mn of plane
+        // n + 1, loaded an iteration ago; the mask of plane n + 2 into mc and
+        // its c_z = +1 directions into fc; the step t + 2 cell of plane
+        // n - 1.  The loop runs it twice per trip with the register sets
+        // swapped (odd: fc = fv1, fn = fv0), so that no register that a load
+        // in flight writes is copied.
+        S fv0[Q], fv1[Q];
+        auto plane = [&](auto odd, uint32_t n, uint32_t& mc, uint32_t& mn) {
+            S (&fc)[Q] = decltype(odd)::value ? fv1 : fv0;
+            S (&fn)[Q] = decltype(odd)::value ? fv0 : fv1;
+            const uint32_t a3 = n % 3u, a2 = n & 1u;
+            if (in_i) mring[a2 * NI + tid] = mc;
+            const bool self_solid = NB ? (mc & 1u) != 0 : mc != 0;
+            mc = load_mask(n + 2u);
+            if (in_i && !self_solid) step1(n, fc, a3, a2);
+            fetch(Flat{}, n + 1u, fn, mn);
+            fetch(Up{}, n + 2u, fc, mc);
+            lds_barrier();
+            if (n >= 2u && in_o) step2(n);
+            lds_barrier();
+        };
+        uint32_t m0 = load_mask(0), m1 = load_mask(1);
+        fetch(All{}, 0, fv0, m0);
+        fetch(Up{}, 1, fv1, m1);
+        for (uint32_t n = 0; n < nplanes; n += 2u) {
+            plane(std::false_type{}, n, m0, m1);
+            if (n + 1u < nplanes) plane(std::true_type{}, n + 1u, m1, m0);
+        }
     }
 }
 
@@ -291,13 +359,14 @@ template <int Q, int V, int BX, bool W>
 bool launch_pull2_bx(const FusedArgs<typename Prec<V>::T>& a, const TwoGeom& g, bool nb, bool mov,
                      const typename Prec<V>::S* A, typename Prec<V>::S* B, const uint8_t* flags,
                      const uint32_t* nbmask, hipStream_t stream) {
-    constexpr int BY = two_rows(Q, (int)sizeof(typename Prec<V>::S), BX);
+    constexpr int sb = (int)sizeof(typename Prec<V>::S), BY = two_rows(Q, sb, BX);
     if constexpr (BY == 0) {
         return false;
     } else {
         const dim3 grid(g.ntx * g.nty * g.nzc), block(two_threads(BX, BY));
-#define PLB_TWO(NBv, MVv) \
-    hipLaunchKernelGGL((k_pull2<Q, V, NBv, MVv, BX, BY, W>), grid, block, 0, stream, A, B, flags, nbmask, a, g)
+#define PLB_TWO(NBv, MVv)                                                                               \
+    hipLaunchKernelGGL((k_pull2<Q, V, NBv, MVv, BX, BY, W, two_split(Q, sb, NBv != 0)>), grid, block, 0, \
+                       stream, A, B, flags, nbmask, a, g)
         if (nb && mov) PLB_TWO(1, true);
         else if (nb) PLB_TWO(1, false);
         else if (mov) PLB_TWO(0, true);

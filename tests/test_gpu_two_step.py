@@ -26,6 +26,11 @@ CASES = [
     (27, 64, 11, 9, "z", True, 4),
     (27, 64, 6, 6, "y", False, 0),
     (19, 96, 9, 7, "z", True, 3),
+    # one and two planes (every z-neighbour is the cell's own plane or the
+    # other one), and one fluid plane between z-walls
+    (19, 64, 6, 1, None, True, 0),
+    (19, 64, 5, 2, "y", True, 1),
+    (27, 64, 4, 3, "z", True, 0),
 ]
 
 
