@@ -299,7 +299,10 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
                               /* XCD sweeps its own contiguous run of tiles; 2    */
                               /* each XCD takes a y-stripe of every z-plane (rows */
                               /* % 8 == 0, else 0); -1 (default) = 2 for the cell */
-                              /* kernels around interior solids, else 0           */
+                              /* kernels around interior solids, else 0.  The     */
+                              /* two-step kernel: 0 workgroup i takes tile i;     */
+                              /* otherwise (default) each XCD takes one of eight  */
+                              /* contiguous runs of its (z-chunk, y-tile) pairs   */
 #define PLB_TUNE_INDEX64 7    /* 1: the cell kernels' 64-bit index instantiations, and */
                               /* the two-step kernel's per-population base pointers,   */
                               /* even where 32-bit byte offsets fit (they run by       */

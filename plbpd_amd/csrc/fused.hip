@@ -665,10 +665,17 @@ namespace {
 
 enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly, kAaEven, kAaOdd, kAaGather, kPull2 };
 
-// Chunks of a two-step launch along z.  A workgroup holds a compute unit's
-// LDS, so the launch runs in rounds of num_cu workgroups, and each chunk of
-// zc planes costs zc + 2 planes of work (the grown planes at its ends): the
-// chunk count minimises rounds x (zc + 2).
+// Chunks of a two-step launch along z, and the workgroup -> tile mapping.  A
+// workgroup holds a compute unit's LDS (or half of it, two_wgs), so the
+// launch runs in rounds of the workgroups resident at once, and each chunk
+// of zc planes costs zc + 2 planes of work (the grown planes at its ends):
+// the chunk count minimises rounds x (zc + 2).  With the XCD mapping
+// (PLB_TUNE_XCD not 0, compute units in multiples of 8) workgroup i runs on
+// XCD i mod 8, and XCD j takes the j-th of eight contiguous runs of the
+// (z-chunk, y-tile) pairs, ordered y-tile first, with every x-tile of each:
+// neighbouring tiles, whose grown borders overlap, then share that XCD's
+// L2.  The runs hold kx = ceil(pairs / 8) pairs (the last may be shorter),
+// so the rounds are counted per XCD.
 TwoGeom two_geom(const plb_state* s) {
     const int bx = fused_two_tile_x(s);
     const int by = two_rows(s->q, (int)s->sbytes, bx);
@@ -677,8 +684,14 @@ TwoGeom two_geom(const plb_state* s) {
     g.wide = (s->tune_index64 || (uint64_t)s->q * s->pstride * s->sbytes > 0xffffffffull) ? 1u : 0u;
     g.ntx = (uint32_t)s->X / (uint32_t)bx;
     g.nty = ((uint32_t)s->Y + (uint32_t)by - 1u) / (uint32_t)by;
-    const uint64_t tiles = (uint64_t)g.ntx * g.nty, Z = (uint64_t)s->Z;
+    const uint64_t ntx = g.ntx, nty = g.nty, Z = (uint64_t)s->Z;
     const uint64_t ncu = (uint64_t)s->num_cu * (uint64_t)two_wgs(bx);  // workgroups resident at once
+    const bool xcd = s->tune_xcd != 0 && s->num_cu % 8 == 0;
+    auto rounds = [&](uint64_t chunks) -> uint64_t {
+        if (!xcd) return (ntx * nty * chunks + ncu - 1) / ncu;
+        const uint64_t kx = (nty * chunks + 7) / 8, per = ncu / 8;
+        return (ntx * kx + per - 1) / per;
+    };
     uint64_t zc = Z;
     if (s->tune_two_zc > 0) {
         zc = std::min<uint64_t>((uint64_t)s->tune_two_zc, Z);
@@ -686,13 +699,16 @@ TwoGeom two_geom(const plb_state* s) {
         uint64_t best = ~0ull;
         for (uint64_t n = 1; n <= 32 && n <= Z; ++n) {
             const uint64_t c = (Z + n - 1) / n, chunks = (Z + c - 1) / c;
-            const uint64_t rounds = (tiles * chunks + ncu - 1) / ncu, cost = rounds * (c + 2);
+            const uint64_t cost = rounds(chunks) * (c + 2);
             if (cost < best) best = cost, zc = c;
         }
     }
     g.zc = (uint32_t)zc;
     g.nzc = (uint32_t)((Z + zc - 1) / zc);
-    g.xcd = g.nty % 8u == 0u ? 1u : 0u;
+    // with fewer than eight pairs some XCDs would get none
+    const uint64_t pairs = nty * g.nzc;
+    g.xcd = xcd && pairs >= 8 ? 1u : 0u;
+    g.kx = (uint32_t)((pairs + 7) / 8);
     return g;
 }
 

@@ -287,10 +287,16 @@ bool launch_pull_row(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts,
 // XCD j (nty % 8 == 0), so that tiles sharing their grown borders share an L2.
 struct TwoGeom {
     uint32_t ntx, nty, nzc, zc;
-    uint32_t xcd;
+    uint32_t xcd;   // 1: XCD j takes the j-th run of kx (z-chunk, y-tile) pairs; 0: workgroup i takes tile i
+    uint32_t kx;
     uint32_t bx;    // 64: one workgroup per compute unit; 32: two
     uint32_t wide;  // population arrays beyond 4 GiB: per-population base pointers (bx 64 only)
 };
+// Workgroups of a two-step launch (with the XCD mapping, eight runs of kx
+// pairs; the workgroups past the last pair return at once).
+__host__ __device__ constexpr uint32_t two_blocks(const TwoGeom& g) {
+    return g.xcd ? 8u * g.ntx * g.kx : g.ntx * g.nty * g.nzc;
+}
 
 // The step t + 1 ring of the two-step kernel.  Directions are grouped by
 // c_z (+1, 0, -1).  The step t + 2 cell at plane z pulls the c_z = +1 group

@@ -188,11 +188,13 @@ __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(cons
     const uint32_t tid = threadIdx.x;
     uint32_t tx, ty, tz;
     if (g.xcd) {
-        const uint32_t j = blockIdx.x & 7u, m = blockIdx.x >> 3, sy = g.nty >> 3;
+        // workgroup i runs on XCD i mod 8, which takes pairs [j kx, (j + 1) kx)
+        const uint32_t j = blockIdx.x & 7u, m = blockIdx.x >> 3;
         tx = m % g.ntx;
-        const uint32_t r = m / g.ntx;
-        ty = j * sy + r % sy;
-        tz = r / sy;
+        const uint32_t pr = j * g.kx + m / g.ntx;
+        if (pr >= g.nty * g.nzc) return;  // past the last pair (the whole workgroup)
+        ty = pr % g.nty;
+        tz = pr / g.nty;
     } else {
         tx = blockIdx.x % g.ntx;
         const uint32_t r = blockIdx.x / g.ntx;
@@ -363,7 +365,7 @@ bool launch_pull2_bx(const FusedArgs<typename Prec<V>::T>& a, const TwoGeom& g, 
     if constexpr (BY == 0) {
         return false;
     } else {
-        const dim3 grid(g.ntx * g.nty * g.nzc), block(two_threads(BX, BY));
+        const dim3 grid(two_blocks(g)), block(two_threads(BX, BY));
 #define PLB_TWO(NBv, MVv)                                                                               \
     hipLaunchKernelGGL((k_pull2<Q, V, NBv, MVv, BX, BY, W, two_split(Q, sb, NBv != 0)>), grid, block, 0, \
                        stream, A, B, flags, nbmask, a, g)

@@ -71,13 +71,15 @@ def test_two_step_matches_oracle(orc, q, X, Y, Z, walls, solids, zc, steps):
         orc.cor_run(o, tau, F, ob, steps, mask)
         orho, ou = orc.cor_moments(o, ob, mask)
         # 64- and 32-cell tiles; per-population base pointers (the form for
-        # arrays beyond 4 GiB, forced by the 64-bit index setting)
-        for bx, wide in ((64, 0), (32, 0), (64, 1)):
+        # arrays beyond 4 GiB, forced by the 64-bit index setting); workgroup
+        # i on tile i instead of the XCD runs
+        for bx, wide, xcd in ((64, 0, -1), (32, 0, -1), (64, 1, -1), (64, 0, 0)):
             if X % bx:
                 continue
             with P.State(q, X, Y, Z, variant, P.FUSED) as st:
                 two_step(st, True, zc, bx)
                 st.set_tuning(capi.TUNE_INDEX64, wide)
+                st.set_tuning(capi.TUNE_XCD, xcd)
                 assert st.scheme() == "pull2", (variant, bx, wide)
                 st.set_boundary(pb, mask)
                 st.initialize(1.0, (0.01, 0.0, 0.0), init, pb)
@@ -86,10 +88,10 @@ def test_two_step_matches_oracle(orc, q, X, Y, Z, walls, solids, zc, steps):
                 raw = st.download_raw()
                 rho, u = st.moments()
             if variant == "f16s":
-                assert np.array_equal(raw.view(np.uint16)[fluid], o["f"].view(np.uint16)[fluid]), (variant, bx, wide)
+                assert np.array_equal(raw.view(np.uint16)[fluid], o["f"].view(np.uint16)[fluid]), (variant, bx, wide, xcd)
             else:
-                assert np.array_equal(raw[fluid], o["f"][fluid]), (variant, bx, wide)
-            assert np.array_equal(rho, orho) and np.array_equal(u, ou), (variant, bx, wide)
+                assert np.array_equal(raw[fluid], o["f"][fluid]), (variant, bx, wide, xcd)
+            assert np.array_equal(rho, orho) and np.array_equal(u, ou), (variant, bx, wide, xcd)
 
 
 @pytest.mark.parametrize("variant,bx", [("f64", 64), ("f32s", 64), ("f16s", 64), ("f32s", 32)])
@@ -256,16 +258,18 @@ def test_staged_calls_pair_into_two_step_launches(orc, variant):
 @pytest.mark.parametrize("q,X,Y,Z,variant", [(19, 128, 100, 77, "f32s"), (19, 192, 41, 33, "f64"),
                                              (27, 384, 40, 29, "f32"), (19, 96, 57, 40, "f32s")])
 def test_two_step_odd_shapes(q, X, Y, Z, variant):
-    """Launch geometries the small oracle cases do not reach: y-tile counts
-    that are not a multiple of 8 (no XCD stripes), ragged last tiles and
-    z-chunks, 384-cell rows, 32-cell tiles (x = 96); against one-step
+    """Launch geometries the small oracle cases do not reach: (z-chunk,
+    y-tile) pair counts that are not a multiple of 8 (a short last XCD run,
+    and workgroups past it), ragged last tiles and z-chunks, 384-cell rows,
+    32-cell tiles (x = 96), and workgroup i on tile i; against one-step
     launches, bit for bit, with solids and a moving lid."""
     ob, pb = bcs("z", lid=(0.02, 0.0, 0.01))
     mask = (np.random.default_rng(31).random((Z, Y, X)) < 0.1).astype(np.uint8)
     out = []
-    for on in (True, False):
+    for on, xcd in ((True, -1), (True, 0), (False, -1)):
         with P.State(q, X, Y, Z, variant, P.FUSED) as st:
             two_step(st, on)
+            st.set_tuning(capi.TUNE_XCD, xcd)
             assert st.scheme() == ("pull2" if on else "pull")
             st.set_boundary(pb, mask)
             st.initialize(1.0, (0.01, 0.0, 0.0), (P.INIT_PERTURBED, 8, 1e-3), pb)
@@ -275,4 +279,5 @@ def test_two_step_odd_shapes(q, X, Y, Z, variant):
     fluid = mask == 0
     fluid[0] = False
     fluid[-1] = False
-    assert np.array_equal(out[0][fluid], out[1][fluid])
+    assert np.array_equal(out[0][fluid], out[2][fluid])
+    assert np.array_equal(out[1][fluid], out[2][fluid])
