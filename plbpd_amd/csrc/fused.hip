@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
         // MOV: an instantiation for boundaries with a moving wall, so that
         // the others carry no wall-term code (compiled in everywhere, the
         // dormant branch cost the 2-byte step about 5%)
-        if (MOV && a.bc.moving && !self_solid) wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
+        if (MOV && a.bc.moving && !self_solid) moving_wall_terms<Q, P>(f, rho, sm, a);
     }
     own_store<Q, V, NTS>(f, B, c, ps, self_solid);
 }
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_aa_even(typename Prec<V>::S* __restr
     const T rho = bgk<Q, P>(f, a, COLLIDE);
     if (MOV && a.bc.moving && !self_solid) {
         const Seams<I> sm = seams<I>(p.x, p.y, p.z, a);
-        wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
+        moving_wall_terms<Q, P>(f, rho, sm, a);
     }
     // solid lanes store zeros, so that every line is written whole
 #pragma unroll
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_aa_odd(typename Prec<V>::S* __restri
 #pragma unroll
     for (int k = 0; k < Q; ++k) f[k] = P::load_raw(*elem(A, slot[k]));
     const T rho = bgk<Q, P>(f, a, COLLIDE);
-    if (MOV && a.bc.moving) wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
+    if (MOV && a.bc.moving) moving_wall_terms<Q, P>(f, rho, sm, a);
     // f*_j goes where population rev(j) was read
 #pragma unroll
     for (int j = 0; j < Q; ++j) *elem(A, slot[rev_k<Q>(j)]) = P::store_raw(f[j]);
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_only(typename Prec<V>::S* __
     if (a.bc.moving) {
         // the first pull-scheme step: the same write-time moving-wall terms as k_pull
         const Seams<uint64_t> sm = seams<uint64_t>(p.x, p.y, p.z, a);
-        wall_terms<Q, 1, P>(f, rho, flags, p.c, sm, a.bc);
+        moving_wall_terms<Q, P>(f, rho, sm, a);
     }
 #pragma unroll
     for (int k = 0; k < Q; ++k) A[(uint64_t)k * a.pstride + p.c] = P::store_raw(f[k]);
@@ -617,6 +617,8 @@ int fused_build_flags(plb_state* s, const plb_boundary* bc, const uint8_t* mask)
                 if (f8) any_solid = true;
                 const bool on[6] = {x == (uint32_t)s->X - 1, x == 0, y == (uint32_t)s->Y - 1,
                                     y == 0, z == (uint32_t)s->Z - 1, z == 0};
+                // the last face wins: wall_terms (fused_kernels.hpp) derives a
+                // wall cell's flag from its coordinates by this same rule
                 for (int f = 0; f < 6; ++f)
                     if (bc->type[f] == kNoslip && on[f] && lattice_moves(s->q, f / 2)) f8 = 1 + f;
                 fl[c] = (uint8_t)f8;
@@ -798,6 +800,7 @@ int launch_kind(plb_state* s, KernelKind kind) {
         a.F1 = (T)s->force[1];
         a.F2 = (T)s->force[2];
         make_fused_bc<T>(s->bc, s->q, &a.bc);
+        fill_wall_cub<Q, T>(a.bc, a.wcub);
         const S* A = (const S*)s->buf[s->cur];
         S* B = (S*)s->buf[s->cur ^ 1];
         const dim3 grid((unsigned)ntiles), block(kBlock);

@@ -23,6 +23,10 @@ struct FusedArgs {
     int32_t nt_load, nt_store;  // non-temporal loads / stores
     T tau, om, omm, F0, F1, F2;
     FusedBC<T> bc;
+    // c_k . u_w for each direction k and axis: u_w the velocity of the wall
+    // face on the side c_k points to along that axis (zero where c_k has no
+    // component along it), in T, as (cx u0 + cy u1) + cz u2 (wall_terms)
+    T wcub[27][3];
 };
 
 namespace {
@@ -231,22 +235,55 @@ __device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Pre
 // else reads that slot.  So the moving-wall term of Boundary.cpp:213-217 is
 // subtracted when the slot is written, with the rho and the expression of the
 // push step (which subtracts it on the way): the two schemes store the same
-// values.  Only wall faces move; interior solids take no term.
+// values.  Only wall faces move; interior solids take no term.  The velocity
+// is that of the wall cell's flag, which fused_build_flags sets from the
+// coordinates alone (1 + the last of the faces x = X - 1, x = 0, y = Y - 1,
+// y = 0, z = Z - 1, z = 0 that holds the cell): so c_k . u_w is the host's
+// a.wcub entry for the last axis along which x + c_k lies on a wall face,
+// taken from the fluid cell's seams with no load (a load of the flag under
+// the branch would make the kernel wait for every load in flight before it,
+// the two-step kernel's next planes among them).
 template <int Q, int K, class P, class I>
-__device__ inline void wall_terms(typename P::T (&f)[Q], typename P::T rho, const uint8_t* __restrict__ flags, I c,
-                                  const Seams<I>& s, const FusedBC<typename P::T>& bc) {
+__device__ inline void wall_terms(typename P::T (&f)[Q], typename P::T rho, const Seams<I>& s,
+                                  const FusedArgs<typename P::T>& a) {
     using T = typename P::T;
     if constexpr (K < Q) {
         constexpr int R = rev_k<Q>(K);
-        constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
+        constexpr int cy = Lat<Q>::cy(K), cz = Lat<Q>::cz(K);
         // neighbour x + c_K (the pull source of direction rev(K)) on a wall face
         if (pull_wall<Q, R>(s)) {
-            const int fl = flags[c + pull_offset<Q, R>(s)];
-            const T cub = (T)cx * bc.vel[fl][0] + (T)cy * bc.vel[fl][1] + (T)cz * bc.vel[fl][2];
+            const bool wz = cz > 0 ? s.wzp : (cz < 0 ? s.wzm : false);
+            const bool wy = cy > 0 ? s.wyp : (cy < 0 ? s.wym : false);
+            const T cub = wz ? a.wcub[K][2] : (wy ? a.wcub[K][1] : a.wcub[K][0]);
             const T term = (T)2 * w_scaled<Q, P>(K) * rho * cub * (T)3;
             f[K] = f[K] - term;
         }
-        wall_terms<Q, K + 1, P>(f, rho, flags, c, s, bc);
+        wall_terms<Q, K + 1, P>(f, rho, s, a);
+    }
+}
+
+// wall_terms for a fluid cell, skipped by the wavefronts none of whose cells
+// is next to a wall face (the vote keeps a branch the compiler would
+// otherwise turn into arithmetic for every cell).
+template <int Q, class P, class I>
+__device__ inline void moving_wall_terms(typename P::T (&f)[Q], typename P::T rho, const Seams<I>& s,
+                                         const FusedArgs<typename P::T>& a) {
+    if (__any(s.wxm || s.wxp || s.wym || s.wyp || s.wzm || s.wzp)) wall_terms<Q, 1, P>(f, rho, s, a);
+}
+
+// The a.wcub table of wall_terms from the wall velocities of bc.
+template <int Q, class T>
+void fill_wall_cub(const FusedBC<T>& bc, T (&wcub)[27][3]) {
+    for (int k = 0; k < 27; ++k)
+        for (int ax = 0; ax < 3; ++ax) wcub[k][ax] = (T)0;
+    for (int k = 0; k < Q; ++k) {
+        const int c[3] = {Lat<Q>::cx(k), Lat<Q>::cy(k), Lat<Q>::cz(k)};
+        for (int ax = 0; ax < 3; ++ax) {
+            if (c[ax] == 0) continue;
+            // face 2 ax is the high side of the axis, 2 ax + 1 the low side
+            const T* u = bc.vel[1 + 2 * ax + (c[ax] > 0 ? 0 : 1)];
+            wcub[k][ax] = (T)c[0] * u[0] + (T)c[1] * u[1] + (T)c[2] * u[2];
+        }
     }
 }
 

@@ -232,12 +232,12 @@ __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(cons
     // Every thread loads, also past the grown tile (its wrapped coordinates
     // stay in the lattice; the values are not used): loads under a branch
     // made the compiler wait for loads in flight where the branches merge.
-    auto load_mask = [&](uint32_t n) -> uint32_t {
+    auto load_mask = [&](uint32_t n) __attribute__((always_inline)) -> uint32_t {
         const I c = ((z0 + n + Z - 1u) % Z) * XY + gy * X + gx;
         return NB ? nbmask[c] : (uint32_t)flags[c];
     };
     // the loads of plane n's directions of group G into fv
-    auto fetch = [&](auto group, uint32_t n, S (&fv)[Q], uint32_t m) {
+    auto fetch = [&](auto group, uint32_t n, S (&fv)[Q], uint32_t m) __attribute__((always_inline)) {
         constexpr int G = decltype(group)::value;
         const uint32_t zi = (z0 + n + Z - 1u) % Z;
         const I c = zi * XY + gy * X + gx;
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(cons
     // Step t + 2 of the tile's cell at plane n - 2 of the chunk (iteration n):
     // from the ring planes of iterations n - 2, n - 1 and n, with the mask of
     // its step t + 1 cell (stored in mring by iteration n - 1).
-    auto step2 = [&](uint32_t n) {
+    auto step2 = [&](uint32_t n) __attribute__((always_inline)) {
         const uint32_t zo = z0 + n - 2u;
         const I c = zo * XY + py * X + px;
         const uint32_t pa = (n + 1u) % 3u, oa = (n + 2u) % 3u, b0 = (n + 1u) & 1u, c1 = n & 1u;
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(cons
             ring_gather<Q, 0, V, NB, NI, RX>(f, ring, li, pa, oa, b0, c1, mo, sm);
             if (!self_solid) {
                 const T rho = bgk<Q, P>(f, a, true);
-                if (MOV && a.bc.moving) wall_terms<Q, 1, P>(f, rho, flags, c, sm, a.bc);
+                if (MOV && a.bc.moving) moving_wall_terms<Q, P>(f, rho, sm, a);
             }
             if (a.nt_store)
                 store_own<Q, V, true, W>(f, B, c, ps, self_solid);
@@ -276,16 +276,12 @@ __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(cons
     };
     // Step t + 1 of this thread's cell at plane n from the loads in fv, into
     // the ring slots a3 (c_z = +1 group) and a2 (the others).
-    auto step1 = [&](uint32_t n, const S (&fv)[Q], uint32_t a3, uint32_t a2) {
+    auto step1 = [&](uint32_t n, const S (&fv)[Q], uint32_t a3, uint32_t a2) __attribute__((always_inline)) {
         T f[Q];
 #pragma unroll
         for (int k = 0; k < Q; ++k) f[k] = P::load_raw(fv[k]);
         const T rho = bgk<Q, P>(f, a, true);
-        if (MOV && a.bc.moving) {
-            const uint32_t zi = (z0 + n + Z - 1u) % Z;
-            const I c = zi * XY + gy * X + gx;
-            wall_terms<Q, 1, P>(f, rho, flags, c, seams<I>(gx, gy, zi, a), a.bc);
-        }
+        if (MOV && a.bc.moving) moving_wall_terms<Q, P>(f, rho, seams<I>(gx, gy, (z0 + n + Z - 1u) % Z, a), a);
         ring_store<Q, 0, V, NI>(f, ring, tid, a3, a2);
     };
 
@@ -298,7 +294,7 @@ __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(cons
         // load).  (Written with one captured register set: the same body with
         // the set passed by reference compiled to longer waits.)
         S fv[Q];
-        auto plane = [&](uint32_t n, uint32_t& mcur, uint32_t& mnext) {
+        auto plane = [&](uint32_t n, uint32_t& mcur, uint32_t& mnext) __attribute__((always_inline)) {
             // (plane numbers wrap modulo Z, so the loads past the chunk's
             // last plane stay in bounds; their values are not used)
             mnext = load_mask(n + 1u);
@@ -331,7 +327,7 @@ __global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(cons
         // swapped (odd: fc = fv1, fn = fv0), so that no register that a load
         // in flight writes is copied.
         S fv0[Q], fv1[Q];
-        auto plane = [&](auto odd, uint32_t n, uint32_t& mc, uint32_t& mn) {
+        auto plane = [&](auto odd, uint32_t n, uint32_t& mc, uint32_t& mn) __attribute__((always_inline)) {
             S (&fc)[Q] = decltype(odd)::value ? fv1 : fv0;
             S (&fn)[Q] = decltype(odd)::value ? fv0 : fv1;
             const uint32_t a3 = n % 3u, a2 = n & 1u;
