@@ -321,7 +321,8 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
                                 /* storage; x a multiple of 32 (of 64 for arrays over */
                                 /* 4 GiB); one population array below 4 GiB; 0 one   */
                                 /* step per launch; -1 (default) automatic: on for    */
-                                /* 4- and 8-byte storage.  The staged calls pair      */
+                                /* 4- and 8-byte storage on lattices of at least 2^18 */
+                                /* cells (2^19 for D3Q27).  The staged calls pair     */
                                 /* whole steps (plb_stream records one step and the   */
                                 /* next plb_stream launches both; any call that reads */
                                 /* or changes the state runs a recorded step first)   */

@@ -939,6 +939,12 @@ bool fused_two_eligible(const plb_state* s) {
     // 2-byte storage ran 1.23x slower with solids (its one-step kernel is
     // closer to the arithmetic limit)
     if (s->tune_two < 0 && s->sbytes == 2) return false;
+    // automatic: lattices of at least 2^18 cells (2^19 for D3Q27).  Below
+    // that the launch has too few tiles and chunks of a few planes, and one
+    // step per launch measured faster (32^3: 0.66-0.91x; 64 x 64 x 32:
+    // 0.67-0.96x; D3Q27 64^3: 0.89-0.94x; D3Q19 64^3 0.94-1.27x)
+    const uint64_t cells = (uint64_t)s->X * (uint64_t)s->Y * (uint64_t)s->Z;
+    if (s->tune_two < 0 && cells < (s->q == 27 ? (1ull << 19) : (1ull << 18))) return false;
     if ((s->q != 19 && s->q != 27) || fused_two_tile_x(s) == 0) return false;
     // offsets within one population array are 32-bit
     if ((uint64_t)s->pstride * s->sbytes > 0xffffffffull) return false;

@@ -135,16 +135,22 @@ def test_two_step_knobs():
             st.set_tuning(capi.TUNE_TWO_TILE_X, 48)
         st.set_tuning(capi.TUNE_TWO_STEP, -1)
         st.set_tuning(capi.TUNE_TWO_ZCHUNK, 0)
+        # automatic: not below 2^18 cells
+        assert st.scheme() == "pull" and st.steps_per_launch() == 1
+        st.set_tuning(capi.TUNE_TWO_STEP, 1)
         assert st.scheme() == "pull2" and st.steps_per_launch() == 2
         st.set_tuning(capi.TUNE_TWO_STEP, 0)
         assert st.scheme() == "pull"
-    # automatic: 4- and 8-byte storage; forced on where a kernel exists
-    with P.State(19, 64, 8, 8, "f16s", P.FUSED) as st:
+    # automatic: 4- and 8-byte storage from 2^18 cells (2^19 for D3Q27);
+    # forced on where a kernel exists
+    with P.State(19, 64, 64, 64, "f16s", P.FUSED) as st:
         assert st.scheme() == "pull"
         st.set_tuning(capi.TUNE_TWO_STEP, 1)
         assert st.scheme() == "pull2"
-    with P.State(19, 64, 8, 8, "f64", P.FUSED) as st:
-        assert st.scheme() == "pull2"
+    for q, dims, v, want in ((19, (64, 64, 64), "f64", "pull2"), (19, (64, 64, 32), "f32s", "pull"),
+                             (27, (64, 64, 64), "f32s", "pull"), (27, (64, 64, 128), "f32s", "pull2")):
+        with P.State(q, *dims, v, P.FUSED) as st:
+            assert st.scheme() == want, (q, dims, v)
     for q, x, v in ((19, 80, "f32s"), (9, 64, "f32s"), (27, 64, "f64")):
         with P.State(q, x, 1 if q == 9 else 8, 8, v, P.FUSED) as st:
             st.set_tuning(capi.TUNE_TWO_STEP, 1)
