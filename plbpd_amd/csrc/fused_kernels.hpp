@@ -320,8 +320,9 @@ bool launch_pull_row(const FusedArgs<typename Prec<V>::T>& a, int vec, bool nts,
                      const uint32_t* nbmask, unsigned nblocks, hipStream_t stream);
 
 // Tiles of a two-step launch: ntx x nty tiles of bx x BY cells in every
-// chunk of zc planes, nzc chunks; xcd = 1 puts y-stripe j of every chunk on
-// XCD j (nty % 8 == 0), so that tiles sharing their grown borders share an L2.
+// chunk of zc planes, nzc chunks; xcd = 1 gives XCD j the j-th of eight runs
+// of kx (z-chunk, y-tile) pairs, so that tiles sharing their grown borders
+// share an L2.
 struct TwoGeom {
     uint32_t ntx, nty, nzc, zc;
     uint32_t xcd;   // 1: XCD j takes the j-th run of kx (z-chunk, y-tile) pairs; 0: workgroup i takes tile i
