@@ -287,3 +287,67 @@ def test_two_step_odd_shapes(q, X, Y, Z, variant):
     fluid[-1] = False
     assert np.array_equal(out[0][fluid], out[2][fluid])
     assert np.array_equal(out[1][fluid], out[2][fluid])
+
+
+def random_case(seed):
+    """A seeded random lattice, boundary, storage variant and launch geometry."""
+    r = np.random.default_rng(1000 + seed)
+    q = int(r.choice([19, 19, 27]))
+    variant = str(r.choice(VARIANTS[q]))
+    X = int(r.choice([32, 64, 96, 128, 160, 192]))
+    Y = int(r.integers(1, 24))
+    Z = int(r.integers(1, 20))
+    axes = "".join(a for a in "xyz" if r.random() < 0.4)
+    if "y" in axes and Y < 3:
+        axes = axes.replace("y", "")
+    if "z" in axes and Z < 3:
+        axes = axes.replace("z", "")
+    faces = {"x": (EAST, WEST), "y": (NORTH, SOUTH), "z": (UP, DOWN)}
+    t = [PERIODIC] * 6
+    v = {}
+    for a in axes:
+        for f in faces[a]:
+            t[f] = NOSLIP
+            v[f] = tuple(float(x) for x in np.round(r.uniform(-0.03, 0.03, 3), 4)) if r.random() < 0.7 else (0.0, 0.0, 0.0)
+    solids = float(r.choice([0.0, 0.05, 0.15]))
+    mask = (r.random((Z, Y, X)) < solids).astype(np.uint8) if solids else None
+    bx = 64 if X % 64 == 0 and r.random() < 0.7 else 32
+    # per-population base pointers are built for 64-cell tiles
+    wide = int(bx == 64 and r.random() < 0.3)
+    knobs = {capi.TUNE_TWO_ZCHUNK: int(r.choice([0, 0, 1, 3, 7])), capi.TUNE_TWO_TILE_X: bx,
+             capi.TUNE_XCD: int(r.choice([-1, 0])), capi.TUNE_INDEX64: wide}
+    steps = int(r.integers(3, 12))
+    return q, variant, X, Y, Z, t, v, mask, knobs, steps
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_two_step_random_cases(orc, seed):
+    """Seeded random lattices (rows of 32 to 192 cells, 1 to 23 rows, 1 to 19
+    planes), walls on random axes with random wall velocities, interior solids,
+    every storage variant, random z-chunk, tile width, tile mapping and index
+    form: two steps per launch against one step per launch, bit for bit on
+    the fluid cells, and the same moments everywhere."""
+    q, variant, X, Y, Z, t, v, mask, knobs, steps = random_case(seed)
+    print(f"\n[case] random {seed}: q={q} {variant} {X}x{Y}x{Z} walls={t} solids={mask is not None} "
+          f"knobs={knobs} steps={steps}", flush=True)
+    pb = P.boundary(t, v)
+    out = []
+    for on in (True, False):
+        with P.State(q, X, Y, Z, variant, P.FUSED) as st:
+            st.set_tuning(capi.TUNE_TWO_STEP, 1 if on else 0)
+            if on:
+                for key, value in knobs.items():
+                    st.set_tuning(key, value)
+                assert st.scheme() == "pull2"
+            st.set_boundary(pb, mask)
+            st.initialize(1.0, (0.01, 0.0, -0.005), (P.INIT_PERTURBED, seed, 1e-3), pb)
+            st.configure(0.58, (1e-5, 2e-6, -1e-6), pb)
+            st.step(steps)
+            out.append((st.download_raw(), st.moments()))
+    fluid = orc.cor_flags(q, X, Y, Z, Boundary(type=t, vel=v), mask) == 0
+    (ra, (rho_a, u_a)), (rb, (rho_b, u_b)) = out
+    if variant == "f16s":
+        assert np.array_equal(ra.view(np.uint16)[fluid], rb.view(np.uint16)[fluid])
+    else:
+        assert np.array_equal(ra[fluid], rb[fluid])
+    assert np.array_equal(rho_a, rho_b) and np.array_equal(u_a, u_b)
