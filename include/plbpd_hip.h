@@ -317,10 +317,9 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
 #define PLB_TUNE_TWO_STEP 9     /* fused pull step: 1 advances two steps per launch   */
                                 /* (one read and one write of the populations per two */
                                 /* steps, the same values bit for bit) wherever a     */
-                                /* kernel exists: D3Q19, and D3Q27 with 2- or 4-byte  */
-                                /* storage; x a multiple of 32 (of 64 for arrays over */
-                                /* 4 GiB); one population array below 4 GiB; 0 one   */
-                                /* step per launch; -1 (default) automatic: on for    */
+                                /* kernel exists: D3Q19 and D3Q27; x a multiple of 32; */
+                                /* one population array below 4 GiB; 0 one step per  */
+                                /* launch; -1 (default) automatic: on for             */
                                 /* 4- and 8-byte storage on lattices of at least 2^18 */
                                 /* cells (2^19 for D3Q27).  The staged calls pair     */
                                 /* whole steps (plb_stream records one step and the   */

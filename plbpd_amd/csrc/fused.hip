@@ -687,7 +687,7 @@ TwoGeom two_geom(const plb_state* s) {
     g.ntx = (uint32_t)s->X / (uint32_t)bx;
     g.nty = ((uint32_t)s->Y + (uint32_t)by - 1u) / (uint32_t)by;
     const uint64_t ntx = g.ntx, nty = g.nty, Z = (uint64_t)s->Z;
-    const uint64_t ncu = (uint64_t)s->num_cu * (uint64_t)two_wgs(bx);  // workgroups resident at once
+    const uint64_t ncu = (uint64_t)s->num_cu * (uint64_t)two_wgs(s->q, (int)s->sbytes, bx);  // resident at once
     const bool xcd = s->tune_xcd != 0 && s->num_cu % 8 == 0;
     auto rounds = [&](uint64_t chunks) -> uint64_t {
         if (!xcd) return (ntx * nty * chunks + ncu - 1) / ncu;
@@ -919,12 +919,7 @@ int launch_kind(plb_state* s, KernelKind kind) {
 }  // namespace
 
 int fused_two_tile_x(const plb_state* s) {
-    // arrays beyond 4 GiB (or PLB_TUNE_INDEX64) take per-population base
-    // pointers, built for 64-cell tiles
-    const bool wide = s->tune_index64 || (uint64_t)s->q * s->pstride * s->sbytes > 0xffffffffull;
-    auto fits = [&](int bx) {
-        return s->X % bx == 0 && two_rows(s->q, (int)s->sbytes, bx) > 0 && (!wide || bx == 64);
-    };
+    auto fits = [&](int bx) { return s->X % bx == 0 && two_rows(s->q, (int)s->sbytes, bx) > 0; };
     if (s->tune_two_bx > 0) return fits(s->tune_two_bx) ? s->tune_two_bx : 0;
     for (int bx : {64, 32})
         if (fits(bx)) return bx;
@@ -935,7 +930,8 @@ bool fused_two_eligible(const plb_state* s) {
     if (s->tune_two == 0 || s->aa || !fused_uses_pull(s) || s->mode != PLB_MODE_FUSED) return false;
     // automatic: 4- and 8-byte storage, where two steps per pass measured
     // 1.40x (D3Q19 256^3 FP32-shifted, 10% solids), 1.57x (walls only), 1.26x
-    // and 1.41x (FP64) and 1.35x (D3Q27 FP32-shifted, walls) faster on MI355X;
+    // and 1.41x (FP64), 1.35x (D3Q27 FP32-shifted, walls) and 1.21x (D3Q27
+    // FP64, walls) faster on MI355X;
     // 2-byte storage ran 1.23x slower with solids (its one-step kernel is
     // closer to the arithmetic limit)
     if (s->tune_two < 0 && s->sbytes == 2) return false;

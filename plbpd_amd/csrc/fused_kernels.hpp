@@ -373,8 +373,6 @@ __host__ __device__ constexpr int zring_planes(int q) {
 // Cells of a grown tile, and threads of its workgroup (one cell each).
 __host__ __device__ constexpr int two_cells(int bx, int by) { return (bx + 2) * (by + 2); }
 __host__ __device__ constexpr int two_threads(int bx, int by) { return (two_cells(bx, by) + 63) / 64 * 64; }
-// Workgroups per compute unit the LDS budget of a tile width allows.
-__host__ __device__ constexpr int two_wgs(int bx) { return bx == 32 ? 2 : 1; }
 // Whether the two-step kernel issues the step t gathers of the c_z = +1
 // directions an iteration earlier than the others (fused_two.hpp): D3Q19
 // with 4-byte storage, and with 8-byte storage around interior solids (nb).
@@ -386,22 +384,30 @@ __host__ __device__ constexpr bool two_split(int q, int sbytes, bool nb) {
     return q == 19 && (sbytes == 4 || (sbytes == 8 && nb));
 }
 
-// Rows per tile for a lattice, a storage width and a tile width: the largest
-// BY whose ring ((bx + 2) x (BY + 2) cells x (zring_planes(q) population
-// planes + two 4-byte mask planes)) fits the LDS share of a workgroup
-// (160 KiB per gfx950 compute unit, split between two_wgs(bx) workgroups),
-// within 1024 threads; 0 where no BY >= 3 fits.  The split kernel with
-// 4-byte storage takes at most 8 rows: its 168 VGPRs allow 12 wavefronts
-// per compute unit (BY <= 9 at 64-cell tiles), and 8-row tiles cut rows of
-// a multiple of 64 cells into a tile count divisible by the 8 XCDs.
-__host__ __device__ constexpr int two_rows(int q, int sbytes, int bx) {
+// Rows per tile for a lattice, a storage width and a tile width within an
+// LDS budget of a workgroup: the largest BY whose ring ((bx + 2) x (BY + 2)
+// cells x (zring_planes(q) population planes + two 4-byte mask planes))
+// fits, within 1024 threads; 0 where no BY >= 3 fits.  The split kernel
+// with 4-byte storage takes at most 8 rows: its 168 VGPRs allow 12
+// wavefronts per compute unit (BY <= 9 at 64-cell tiles), and 9 rows ran
+// slower.
+__host__ __device__ constexpr int two_rows_in(int q, int sbytes, int bx, int budget) {
     int best = 0;
     const int cap = two_split(q, sbytes, true) && sbytes == 4 ? 8 : 30;
     for (int by = 1; by <= cap; ++by)
-        if (two_cells(bx, by) * (zring_planes(q) * sbytes + 8) <= 160 * 1024 / two_wgs(bx) &&
-            two_cells(bx, by) <= 1024)
-            best = by;
+        if (two_cells(bx, by) * (zring_planes(q) * sbytes + 8) <= budget && two_cells(bx, by) <= 1024) best = by;
     return best >= 3 ? best : 0;
+}
+// Workgroups per compute unit (160 KiB of LDS on gfx950): a 64-cell tile
+// takes the compute unit's LDS; a 32-cell tile half of it where a tile of 3
+// rows or more fits there, else all of it (D3Q27 with 8-byte storage, whose
+// ring fits no 64-cell tile).
+__host__ __device__ constexpr int two_wgs(int q, int sbytes, int bx) {
+    return bx == 32 && two_rows_in(q, sbytes, bx, 80 * 1024) > 0 ? 2 : 1;
+}
+// Rows per tile in the LDS share of a workgroup.
+__host__ __device__ constexpr int two_rows(int q, int sbytes, int bx) {
+    return two_rows_in(q, sbytes, bx, 160 * 1024 / two_wgs(q, sbytes, bx));
 }
 
 // Two pull steps in one pass (fused_two.hpp), tiles of g.bx x two_rows(Q, S,

@@ -172,7 +172,7 @@ __device__ inline void lds_barrier() {
 }
 
 template <int Q, int V, int NB, bool MOV, int BX, int BY, bool W, bool SPLIT>
-__global__ __launch_bounds__(two_threads(BX, BY), two_wgs(BX)) void k_pull2(const typename Prec<V>::S* __restrict__ A,
+__global__ __launch_bounds__(two_threads(BX, BY), two_wgs(Q, (int)sizeof(typename Prec<V>::S), BX)) void k_pull2(const typename Prec<V>::S* __restrict__ A,
                                                            typename Prec<V>::S* __restrict__ B,
                                                            const uint8_t* __restrict__ flags,
                                                            const uint32_t* __restrict__ nbmask,
@@ -381,6 +381,7 @@ bool launch_pull2(const FusedArgs<typename Prec<V>::T>& a, const TwoGeom& g, boo
     if (g.bx == 64 && !g.wide) return launch_pull2_bx<Q, V, 64, false>(a, g, nb, mov, A, B, flags, nbmask, stream);
     if (g.bx == 64) return launch_pull2_bx<Q, V, 64, true>(a, g, nb, mov, A, B, flags, nbmask, stream);
     if (g.bx == 32 && !g.wide) return launch_pull2_bx<Q, V, 32, false>(a, g, nb, mov, A, B, flags, nbmask, stream);
+    if (g.bx == 32) return launch_pull2_bx<Q, V, 32, true>(a, g, nb, mov, A, B, flags, nbmask, stream);
     return false;
 }
 

@@ -1,5 +1,5 @@
 // Two pull steps per pass (fused_two.hpp) for the D3Q27 lattice, every storage variant
-// (8-byte storage has no kernel: its ring leaves too few rows per tile).
+// (8-byte storage in 32-cell tiles only: its ring leaves a 64-cell tile too few rows).
 #include "fused_two.hpp"
 
 namespace plb {

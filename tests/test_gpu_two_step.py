@@ -13,7 +13,7 @@ from oracle.pyoracle import Boundary, NOSLIP, PERIODIC, EAST, WEST, NORTH, SOUTH
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = {19: ["f64", "f32", "f64a32", "f32s", "f16s"], 27: ["f32", "f32s", "f16s"]}
+VARIANTS = {19: ["f64", "f32", "f64a32", "f32s", "f16s"], 27: ["f64", "f32", "f32s", "f16s"]}
 
 # q, X, Y, Z, walls ("z": up/down, "y": north/south, "x": east/west, None), solids, z-chunk
 CASES = [
@@ -73,9 +73,11 @@ def test_two_step_matches_oracle(orc, q, X, Y, Z, walls, solids, zc, steps):
         # 64- and 32-cell tiles; per-population base pointers (the form for
         # arrays beyond 4 GiB, forced by the 64-bit index setting); workgroup
         # i on tile i instead of the XCD runs
-        for bx, wide, xcd in ((64, 0, -1), (32, 0, -1), (64, 1, -1), (64, 0, 0)):
+        for bx, wide, xcd in ((64, 0, -1), (32, 0, -1), (64, 1, -1), (32, 1, -1), (64, 0, 0)):
             if X % bx:
                 continue
+            if q == 27 and variant == "f64" and bx == 64:
+                continue  # its ring fits 32-cell tiles only
             with P.State(q, X, Y, Z, variant, P.FUSED) as st:
                 two_step(st, True, zc, bx)
                 st.set_tuning(capi.TUNE_INDEX64, wide)
@@ -151,10 +153,16 @@ def test_two_step_knobs():
                              (27, (64, 64, 64), "f32s", "pull"), (27, (64, 64, 128), "f32s", "pull2")):
         with P.State(q, *dims, v, P.FUSED) as st:
             assert st.scheme() == want, (q, dims, v)
-    for q, x, v in ((19, 80, "f32s"), (9, 64, "f32s"), (27, 64, "f64")):
+    for q, x, v in ((19, 80, "f32s"), (9, 64, "f32s")):
         with P.State(q, x, 1 if q == 9 else 8, 8, v, P.FUSED) as st:
             st.set_tuning(capi.TUNE_TWO_STEP, 1)
             assert st.scheme() == "pull", (q, x, v)
+    # D3Q27 with 8-byte storage: 32-cell tiles only
+    with P.State(27, 64, 8, 8, "f64", P.FUSED) as st:
+        st.set_tuning(capi.TUNE_TWO_STEP, 1)
+        assert st.scheme() == "pull2"
+        st.set_tuning(capi.TUNE_TWO_TILE_X, 64)
+        assert st.scheme() == "pull"
     with P.State(19, 64, 8, 8, "f32s", P.FUSED) as st:
         st.set_tuning(capi.TUNE_SCHEME, capi.SCHEME_PUSH)
         assert st.scheme() == "push"
@@ -311,9 +319,8 @@ def random_case(seed):
             v[f] = tuple(float(x) for x in np.round(r.uniform(-0.03, 0.03, 3), 4)) if r.random() < 0.7 else (0.0, 0.0, 0.0)
     solids = float(r.choice([0.0, 0.05, 0.15]))
     mask = (r.random((Z, Y, X)) < solids).astype(np.uint8) if solids else None
-    bx = 64 if X % 64 == 0 and r.random() < 0.7 else 32
-    # per-population base pointers are built for 64-cell tiles
-    wide = int(bx == 64 and r.random() < 0.3)
+    bx = 64 if X % 64 == 0 and r.random() < 0.7 and (q, variant) != (27, "f64") else 32
+    wide = int(r.random() < 0.2)
     knobs = {capi.TUNE_TWO_ZCHUNK: int(r.choice([0, 0, 1, 3, 7])), capi.TUNE_TWO_TILE_X: bx,
              capi.TUNE_XCD: int(r.choice([-1, 0])), capi.TUNE_INDEX64: wide}
     steps = int(r.integers(3, 12))
