@@ -327,8 +327,10 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
                                 /* or changes the state runs a recorded step first)   */
 #define PLB_TUNE_TWO_ZCHUNK 10  /* planes per z-chunk of a two-step launch (0: chosen */
                                 /* from the grid and the compute-unit count)          */
-#define PLB_TUNE_TWO_TILE_X 11  /* two-step tile width: 64 cells (one workgroup per   */
-                                /* compute unit) or 32 (two); 0 automatic             */
+#define PLB_TUNE_TWO_TILE_X 11  /* two-step tile width: 64 or 32 cells (one          */
+                                /* workgroup per compute unit either way, 32-cell     */
+                                /* tiles taller); 0 automatic: 32 for 8-byte storage, */
+                                /* else 64                                            */
 int plb_set_tuning(plb_state* s, int key, int value);
 /* The streaming scheme the next fused steps will use: PLB_SCHEME_PUSH,
  * PLB_SCHEME_PULL, PLB_SCHEME_PULL_TWO (pull, two steps per launch, for

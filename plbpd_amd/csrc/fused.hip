@@ -668,8 +668,8 @@ namespace {
 enum KernelKind { kPush, kPull, kGather, kStreamOnly, kCollideOnly, kAaEven, kAaOdd, kAaGather, kPull2 };
 
 // Chunks of a two-step launch along z, and the workgroup -> tile mapping.  A
-// workgroup holds a compute unit's LDS (or half of it, two_wgs), so the
-// launch runs in rounds of the workgroups resident at once, and each chunk
+// workgroup holds a compute unit's LDS, so the launch runs in rounds of the
+// workgroups resident at once (one per compute unit), and each chunk
 // of zc planes costs zc + 2 planes of work (the grown planes at its ends):
 // the chunk count minimises rounds x (zc + 2).  With the XCD mapping
 // (PLB_TUNE_XCD not 0, compute units in multiples of 8) workgroup i runs on
@@ -687,7 +687,7 @@ TwoGeom two_geom(const plb_state* s) {
     g.ntx = (uint32_t)s->X / (uint32_t)bx;
     g.nty = ((uint32_t)s->Y + (uint32_t)by - 1u) / (uint32_t)by;
     const uint64_t ntx = g.ntx, nty = g.nty, Z = (uint64_t)s->Z;
-    const uint64_t ncu = (uint64_t)s->num_cu * (uint64_t)two_wgs(s->q, (int)s->sbytes, bx);  // resident at once
+    const uint64_t ncu = (uint64_t)s->num_cu;  // workgroups resident at once (one per compute unit)
     const bool xcd = s->tune_xcd != 0 && s->num_cu % 8 == 0;
     auto rounds = [&](uint64_t chunks) -> uint64_t {
         if (!xcd) return (ntx * nty * chunks + ncu - 1) / ncu;
@@ -921,7 +921,8 @@ int launch_kind(plb_state* s, KernelKind kind) {
 int fused_two_tile_x(const plb_state* s) {
     auto fits = [&](int bx) { return s->X % bx == 0 && two_rows(s->q, (int)s->sbytes, bx) > 0; };
     if (s->tune_two_bx > 0) return fits(s->tune_two_bx) ? s->tune_two_bx : 0;
-    for (int bx : {64, 32})
+    const int pref = two_tile_pref(s->q, (int)s->sbytes);
+    for (int bx : {pref, 96 - pref})
         if (fits(bx)) return bx;
     return 0;
 }

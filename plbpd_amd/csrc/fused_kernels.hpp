@@ -384,31 +384,27 @@ __host__ __device__ constexpr bool two_split(int q, int sbytes, bool nb) {
     return q == 19 && (sbytes == 4 || (sbytes == 8 && nb));
 }
 
-// Rows per tile for a lattice, a storage width and a tile width within an
-// LDS budget of a workgroup: the largest BY whose ring ((bx + 2) x (BY + 2)
-// cells x (zring_planes(q) population planes + two 4-byte mask planes))
-// fits, within 1024 threads; 0 where no BY >= 3 fits.  The split kernel
-// with 4-byte storage takes at most 8 rows: its 168 VGPRs allow 12
-// wavefronts per compute unit (BY <= 9 at 64-cell tiles), and 9 rows ran
-// slower.
-__host__ __device__ constexpr int two_rows_in(int q, int sbytes, int bx, int budget) {
+// Rows per tile for a lattice, a storage width and a tile width: the largest
+// BY whose ring ((bx + 2) x (BY + 2) cells x (zring_planes(q) population
+// planes + two 4-byte mask planes)) fits the LDS of a compute unit (160 KiB
+// on gfx950; a workgroup takes all of it), within 1024 threads; 0 where no
+// BY >= 3 fits.  The split kernel with 4-byte storage takes at most 704
+// cells: its 168 VGPRs allow 12 wavefronts per compute unit, and 64 x 9
+// tiles (768 threads) ran slower than 64 x 8.
+__host__ __device__ constexpr int two_rows(int q, int sbytes, int bx) {
     int best = 0;
-    const int cap = two_split(q, sbytes, true) && sbytes == 4 ? 8 : 30;
-    for (int by = 1; by <= cap; ++by)
-        if (two_cells(bx, by) * (zring_planes(q) * sbytes + 8) <= budget && two_cells(bx, by) <= 1024) best = by;
+    const int maxc = two_split(q, sbytes, true) && sbytes == 4 ? 704 : 1024;
+    for (int by = 1; by <= 40; ++by)
+        if (two_cells(bx, by) * (zring_planes(q) * sbytes + 8) <= 160 * 1024 && two_cells(bx, by) <= maxc) best = by;
     return best >= 3 ? best : 0;
 }
-// Workgroups per compute unit (160 KiB of LDS on gfx950): a 64-cell tile
-// takes the compute unit's LDS; a 32-cell tile half of it where a tile of 3
-// rows or more fits there, else all of it (D3Q27 with 8-byte storage, whose
-// ring fits no 64-cell tile).
-__host__ __device__ constexpr int two_wgs(int q, int sbytes, int bx) {
-    return bx == 32 && two_rows_in(q, sbytes, bx, 80 * 1024) > 0 ? 2 : 1;
-}
-// Rows per tile in the LDS share of a workgroup.
-__host__ __device__ constexpr int two_rows(int q, int sbytes, int bx) {
-    return two_rows_in(q, sbytes, bx, 160 * 1024 / two_wgs(q, sbytes, bx));
-}
+// The tile width a lattice and storage width prefer: 32 for 8-byte storage,
+// whose taller tile recomputes fewer grown-border cells (D3Q19 32 x 11
+// against 64 x 5: 6-13% faster from 128^3 to 384^3; D3Q27 fits 32-cell tiles
+// only), 64 otherwise (D3Q19 4-byte 32 x 18 against 64 x 8: 19-22% slower;
+// D3Q27 4-byte 32 x 16 against 64 x 7: 7-10% faster at 256^3 but 2.5%
+// slower at 384^3).
+__host__ __device__ constexpr int two_tile_pref(int q, int sbytes) { return (void)q, sbytes == 8 ? 32 : 64; }
 
 // Two pull steps in one pass (fused_two.hpp), tiles of g.bx x two_rows(Q, S,
 // g.bx) cells.  Returns false where no kernel exists for (Q, V, g.bx, g.wide).

@@ -172,7 +172,7 @@ __device__ inline void lds_barrier() {
 }
 
 template <int Q, int V, int NB, bool MOV, int BX, int BY, bool W, bool SPLIT>
-__global__ __launch_bounds__(two_threads(BX, BY), two_wgs(Q, (int)sizeof(typename Prec<V>::S), BX)) void k_pull2(const typename Prec<V>::S* __restrict__ A,
+__global__ __launch_bounds__(two_threads(BX, BY), 1) void k_pull2(const typename Prec<V>::S* __restrict__ A,
                                                            typename Prec<V>::S* __restrict__ B,
                                                            const uint8_t* __restrict__ flags,
                                                            const uint32_t* __restrict__ nbmask,
