@@ -324,6 +324,7 @@ def random_case(seed):
     knobs = {capi.TUNE_TWO_ZCHUNK: int(r.choice([0, 0, 1, 3, 7])), capi.TUNE_TWO_TILE_X: bx,
              capi.TUNE_XCD: int(r.choice([-1, 0])), capi.TUNE_INDEX64: wide}
     steps = int(r.integers(3, 12))
+    knobs[capi.TUNE_NT_STORE] = int(r.random() < 0.5)
     return q, variant, X, Y, Z, t, v, mask, knobs, steps
 
 
@@ -331,9 +332,9 @@ def random_case(seed):
 def test_two_step_random_cases(orc, seed):
     """Seeded random lattices (rows of 32 to 192 cells, 1 to 23 rows, 1 to 19
     planes), walls on random axes with random wall velocities, interior solids,
-    every storage variant, random z-chunk, tile width, tile mapping and index
-    form: two steps per launch against one step per launch, bit for bit on
-    the fluid cells, and the same moments everywhere."""
+    every storage variant, random z-chunk, tile width, tile mapping, index
+    form and store kind: two steps per launch against one step per launch,
+    bit for bit on the fluid cells, and the same moments everywhere."""
     q, variant, X, Y, Z, t, v, mask, knobs, steps = random_case(seed)
     print(f"\n[case] random {seed}: q={q} {variant} {X}x{Y}x{Z} walls={t} solids={mask is not None} "
           f"knobs={knobs} steps={steps}", flush=True)
