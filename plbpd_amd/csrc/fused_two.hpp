@@ -152,10 +152,17 @@ __device__ inline void ring_gather(typename Prec<V>::T (&f)[Q], const typename P
         constexpr int R = rev_k<Q>(K);
         constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K);
         constexpr int g = Z::grp(K), gr = Z::grp(R);
-        const bool solid = NB ? ((m >> R) & 1u) != 0 : pull_wall<Q, K>(s);
         const uint32_t moved = ring_at<Q, K, NI>(g == 0 ? pa : (g == 1 ? b0 : c1)) + li - (uint32_t)(cx + cy * RX);
         const uint32_t own = ring_at<Q, R, NI>(gr == 0 ? oa : b0) + li;
-        f[K] = Prec<V>::load_raw(ring[solid ? own : moved]);
+        if constexpr (NB != 0) {
+            // own - moved is uniform (li cancels) and, like every ring index,
+            // far inside the signed 24-bit range: one 24-bit multiply-add by
+            // the mask bit instead of a compare and a select
+            const int d = __mul24((int)((m >> R) & 1u), (int)(own - moved));
+            f[K] = Prec<V>::load_raw(ring[moved + (uint32_t)d]);
+        } else {
+            f[K] = Prec<V>::load_raw(ring[pull_wall<Q, K>(s) ? own : moved]);
+        }
         ring_gather<Q, K + 1, V, NB, NI, RX>(f, ring, li, pa, oa, b0, c1, m, s);
     }
 }
