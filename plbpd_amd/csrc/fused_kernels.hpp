@@ -327,8 +327,8 @@ struct TwoGeom {
     uint32_t ntx, nty, nzc, zc;
     uint32_t xcd;   // 1: XCD j takes the j-th run of kx (z-chunk, y-tile) pairs; 0: workgroup i takes tile i
     uint32_t kx;
-    uint32_t bx;    // 64: one workgroup per compute unit; 32: two
-    uint32_t wide;  // population arrays beyond 4 GiB: per-population base pointers (bx 64 only)
+    uint32_t bx;    // tile width, 64 or 32 cells (one workgroup per compute unit either way)
+    uint32_t wide;  // population arrays beyond 4 GiB: per-population base pointers
 };
 // Workgroups of a two-step launch (with the XCD mapping, eight runs of kx
 // pairs; the workgroups past the last pair return at once).
@@ -341,7 +341,12 @@ __host__ __device__ constexpr uint32_t two_blocks(const TwoGeom& g) {
 // from plane z - 1, the c_z = 0 group from plane z and the c_z = -1 group
 // from plane z + 1, and its own reverse slots (bounce-back) from plane z: a
 // plane's c_z = +1 group is read up to two planes after it is computed, the
-// other groups up to one, so the groups keep 3, 2 and 2 planes.
+// other groups up to one, so the groups keep 3, 2 and 2 planes.  (One plane
+// more per group would let the step t + 1 cells of the next plane be written
+// while the step t + 2 cells of this one are read, with one workgroup barrier
+// per plane instead of two; for D3Q19 FP32-shifted its 64 x 7 tiles ran 354 /
+// 309 us per step at 256^3 with solids / walls only, against 341 / 288 for
+// two barriers and 64 x 8 tiles.)
 template <int Q>
 struct ZRing {
     __host__ __device__ static constexpr int grp(int k) {
@@ -389,8 +394,9 @@ __host__ __device__ constexpr bool two_split(int q, int sbytes, bool nb) {
 // planes + two 4-byte mask planes)) fits the LDS of a compute unit (160 KiB
 // on gfx950; a workgroup takes all of it), within 1024 threads; 0 where no
 // BY >= 3 fits.  The split kernel with 4-byte storage takes at most 704
-// cells: its 168 VGPRs allow 12 wavefronts per compute unit, and 64 x 9
-// tiles (768 threads) ran slower than 64 x 8.
+// cells: 64 x 9 tiles (768 threads) ran slower than 64 x 8, and so did
+// 64 x 11 (858 threads in 128 VGPRs: D3Q19 256^3 FP32-shifted 356 / 307
+// against 341 / 288 us per step with solids / walls only).
 __host__ __device__ constexpr int two_rows(int q, int sbytes, int bx) {
     int best = 0;
     const int maxc = two_split(q, sbytes, true) && sbytes == 4 ? 704 : 1024;

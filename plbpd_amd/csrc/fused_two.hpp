@@ -91,26 +91,34 @@ __device__ inline void gather_mask(typename Prec<V>::S (&fv)[Q], const typename 
         if constexpr (W) {
             fv[K] = *pop_elem<W>(A, solid ? R : (uint32_t)K, ps, solid ? c : c + pull_offset<Q, K>(s));
         } else {
-            // select between two offsets whose population parts are uniform
-            // (a selected population number times pstride would be a per-lane
-            // multiply)
+            // byte offsets (c, ps and the seams of s in bytes): the population
+            // parts are uniform (a selected population number times pstride
+            // would be a per-lane multiply) and the cell parts shared between
+            // directions; the mask bit, sign-extended to all ones, selects
+            // bit by bit
+            using S = typename Prec<V>::S;
+            (void)solid;
             const uint32_t moved = (uint32_t)K * ps + (c + pull_offset<Q, K>(s));
             const uint32_t own = R * ps + c;
-            fv[K] = *elem(A, solid ? own : moved);
+            const uint32_t sel = (uint32_t)__builtin_amdgcn_sbfe((int)m, (int)R, 1);
+            fv[K] = *reinterpret_cast<const S*>(reinterpret_cast<const char*>(A) + (moved ^ ((moved ^ own) & sel)));
         }
         gather_mask<Q, K + 1, V, W, G>(fv, A, c, ps, m, s);
     }
 }
 
 // Step t + 2 of a cell into its own slots (zeros for a cell that is not fluid).
-template <int Q, int V, bool NT, bool W>
+// Each population's base pointer is a scalar and the cell's byte offset is
+// shared by all of them, so a store takes no address arithmetic of its own
+// (one 32-bit offset per population would be a vector add per store).
+template <int Q, int V, bool NT>
 __device__ inline void store_own(const typename Prec<V>::T (&f)[Q], typename Prec<V>::S* __restrict__ B, uint32_t c,
                                  uint32_t ps, bool zero) {
     using P = Prec<V>;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const typename P::S v = P::store_raw(zero ? (typename P::T)0 : f[k]);
-        typename P::S* p = pop_elem<W>(B, (uint32_t)k, ps, c);
+        typename P::S* p = pop_elem<true>(B, (uint32_t)k, ps, c);
         if (NT)
             __builtin_nontemporal_store(v, p);
         else
@@ -249,10 +257,17 @@ __global__ __launch_bounds__(two_threads(BX, BY), 1) void k_pull2(const typename
         const uint32_t zi = (z0 + n + Z - 1u) % Z;
         const I c = zi * XY + gy * X + gx;
         const Seams<I> sm = seams<I>(gx, gy, zi, a);
-        if (NB)
+        if (NB && !W) {
+            // in bytes: one vector add per cell part, none per direction
+            constexpr I sz = (I)sizeof(S);
+            Seams<I> sb = sm;
+            sb.xm *= sz, sb.xp *= sz, sb.ym *= sz, sb.yp *= sz, sb.zm *= sz, sb.zp *= sz;
+            gather_mask<Q, 0, V, W, G>(fv, A, c * sz, ps * sz, m, sb);
+        } else if (NB) {
             gather_mask<Q, 0, V, W, G>(fv, A, c, ps, m, sm);
-        else
+        } else {
             gather_walls<Q, 0, V, W, G>(fv, A, c, ps, sm);
+        }
     };
     using All = std::integral_constant<int, kAll>;
     using Up = std::integral_constant<int, kUp>;
@@ -276,9 +291,9 @@ __global__ __launch_bounds__(two_threads(BX, BY), 1) void k_pull2(const typename
                 if (MOV && a.bc.moving) moving_wall_terms<Q, P>(f, rho, sm, a);
             }
             if (a.nt_store)
-                store_own<Q, V, true, W>(f, B, c, ps, self_solid);
+                store_own<Q, V, true>(f, B, c, ps, self_solid);
             else
-                store_own<Q, V, false, W>(f, B, c, ps, self_solid);
+                store_own<Q, V, false>(f, B, c, ps, self_solid);
         }
     };
     // Step t + 1 of this thread's cell at plane n from the loads in fv, into
