@@ -176,8 +176,13 @@ __global__ __launch_bounds__(kBlock) void k_pull(const typename Prec<V>::S* __re
         pull_gather<Q, 0, V, NB, false>(f, A, c, ps, m, sm);
     } else {
         // the addresses do not depend on the flag: every load is issued
-        // before the first wait (a wavefront of wall cells loads in vain)
-        pull_gather<Q, 0, V, NB, false>(f, A, c, ps, m, sm);
+        // before the first wait (a wavefront of wall cells loads in vain);
+        // a wavefront none of whose cells borders a wall face pulls every
+        // population without the bounce-back select
+        if (__any(sm.wxm || sm.wxp || sm.wym || sm.wyp || sm.wzm || sm.wzp))
+            pull_gather<Q, 0, V, NB, false>(f, A, c, ps, m, sm);
+        else
+            pull_gather<Q, 0, V, NB, false, false>(f, A, c, ps, m, sm);
         self_solid = flags[c] != 0;
         if (__all(self_solid)) return;
     }

@@ -178,21 +178,62 @@ __device__ inline S* elem(S* base, I i) {
         return base + i;
 }
 
-template <int Q, int K, int V, int NB, bool NT, class I>
+// The in-plane (x, y) and the z part of pull_offset.
+template <int Q, int K, class I>
+__device__ inline I pull_offset_xy(const Seams<I>& s) {
+    constexpr int cx = Lat<Q>::cx(K), cy = Lat<Q>::cy(K);
+    I d = 0;
+    if (cx > 0) d += s.xm;
+    if (cx < 0) d += s.xp;
+    if (cy > 0) d += s.ym;
+    if (cy < 0) d += s.yp;
+    return d;
+}
+template <int Q, int K, class I>
+__device__ inline I pull_offset_z(const Seams<I>& s) {
+    constexpr int cz = Lat<Q>::cz(K);
+    return cz > 0 ? s.zm : (cz < 0 ? s.zp : (I)0);
+}
+
+// With 32-bit indices (every byte offset of the array fits 32 bits, so every
+// element offset fits 31) the gathers and stores below split an address into
+// a part that is uniform across a workgroup (the population times pstride,
+// and the z shift: a workgroup's cells share their plane) and a per-cell
+// part, so that the per-cell vector work is shared between directions
+// instead of repeated for each.
+// BB = false: no source is solid (a wavefront away from the wall faces).
+template <int Q, int K, int V, int NB, bool NT, bool BB = true, class I>
 __device__ inline void pull_gather(typename Prec<V>::T (&f)[Q], const typename Prec<V>::S* __restrict__ A,
                                    I c, I ps, uint32_t m, const Seams<I>& s) {
     if constexpr (K < Q) {
         using P = Prec<V>;
+        using S = typename P::S;
         constexpr int R = rev_k<Q>(K);
         // source x - c_k; it is solid when neighbour rev(k) is
         const bool solid = NB ? ((m >> R) & 1u) != 0 : pull_wall<Q, K>(s);
-        const I moved = (I)K * ps + (c + pull_offset<Q, K>(s));
-        const I own = (I)R * ps + c;
-        // branch-free select (a conditional here becomes divergent branches)
-        const I src = moved ^ ((moved ^ own) & ((I)0 - (I)solid));
-        const typename P::S* p = elem(A, src);
+        const S* p;
+        if constexpr (!BB && sizeof(I) == 4) {
+            (void)solid;
+            const int64_t zo = (int32_t)pull_offset_z<Q, K>(s);
+            p = elem(A + ((int64_t)K * ps + zo), c + pull_offset_xy<Q, K>(s));
+        } else if constexpr (!BB) {
+            (void)solid;
+            p = A + ((I)K * ps + (c + pull_offset<Q, K>(s)));
+        } else if constexpr (sizeof(I) == 4) {
+            // byte offsets from A, selected bit by bit
+            constexpr uint32_t sz = sizeof(S);
+            const uint32_t moved = ((uint32_t)K * ps + pull_offset_z<Q, K>(s)) * sz + (c + pull_offset_xy<Q, K>(s)) * sz;
+            const uint32_t own = (uint32_t)R * ps * sz + c * sz;
+            const uint32_t src = moved ^ ((moved ^ own) & (0u - (uint32_t)solid));
+            p = reinterpret_cast<const S*>(reinterpret_cast<const char*>(A) + src);
+        } else {
+            const I moved = (I)K * ps + (c + pull_offset<Q, K>(s));
+            const I own = (I)R * ps + c;
+            // branch-free select (a conditional here becomes divergent branches)
+            p = A + (moved ^ ((moved ^ own) & ((I)0 - (I)solid)));
+        }
         f[K] = NT ? P::load_raw(__builtin_nontemporal_load(p)) : P::load_raw(*p);
-        pull_gather<Q, K + 1, V, NB, NT>(f, A, c, ps, m, s);
+        pull_gather<Q, K + 1, V, NB, NT, BB>(f, A, c, ps, m, s);
     }
 }
 
@@ -207,8 +248,17 @@ __device__ inline void pull_gather_both(typename Prec<V>::S (&fv)[Q], typename P
                                         const typename Prec<V>::S* __restrict__ A, I c, I ps,
                                         const Seams<I>& s) {
     if constexpr (K < Q) {
-        fv[K] = *elem(A, (I)K * ps + (c + pull_offset<Q, K>(s)));
-        if constexpr (K > 0) own[K] = *elem(A, (I)rev_k<Q>(K) * ps + c);
+        if constexpr (sizeof(I) == 4) {
+            // population base and z shift as one uniform pointer; the in-plane
+            // offset (shared by the directions with the same c_x, c_y) as the
+            // vector operand
+            const int64_t zo = (int32_t)pull_offset_z<Q, K>(s);
+            fv[K] = *elem(A + ((int64_t)K * ps + zo), c + pull_offset_xy<Q, K>(s));
+            if constexpr (K > 0) own[K] = *elem(A + (uint64_t)rev_k<Q>(K) * ps, c);
+        } else {
+            fv[K] = *elem(A, (I)K * ps + (c + pull_offset<Q, K>(s)));
+            if constexpr (K > 0) own[K] = *elem(A, (I)rev_k<Q>(K) * ps + c);
+        }
         pull_gather_both<Q, K + 1, V>(fv, own, A, c, ps, s);
     }
 }
@@ -221,7 +271,14 @@ __device__ inline void own_store(const typename Prec<V>::T (&f)[Q], typename Pre
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const typename P::S v = P::store_raw(zero ? (T)0 : f[k]);
-        typename P::S* p = elem(B, (I)k * ps + c);
+        // the population's base pointer is uniform, so every store takes the
+        // same vector offset; the empty asm keeps the base a scalar operand
+        // (else the compiler chains the full 64-bit addresses in vector
+        // registers, one 64-bit add per store)
+        using GS = __attribute__((address_space(1))) typename P::S;
+        uint64_t base = (uint64_t)(B + (uint64_t)k * ps);
+        asm("" : "+s"(base));
+        GS* p = reinterpret_cast<GS*>(base + (uint64_t)(c * (I)sizeof(typename P::S)));
         if (NT)
             __builtin_nontemporal_store(v, p);
         else
