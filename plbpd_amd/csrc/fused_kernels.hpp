@@ -438,7 +438,7 @@ __host__ __device__ constexpr int two_threads(int bx, int by) { return (two_cell
 // Whether the two-step kernel issues the step t gathers of the c_z = +1
 // directions an iteration earlier than the others (fused_two.hpp): D3Q19
 // with 4-byte storage, and with 8-byte storage around interior solids (nb).
-// The values held for the extra iteration fit there without spills (at 168
+// The values held for the extra iteration fit there without spills (136
 // VGPRs with 4-byte storage, 256 with 8-byte); D3Q27 (9 such directions) and
 // 2-byte storage (13-row tiles at 128 VGPRs) spilled, and 8-byte storage
 // without interior solids measured 1% slower.
