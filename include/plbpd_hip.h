@@ -318,7 +318,7 @@ int plb_bench_pattern(int device, int q, int x, int y, int z, int sbytes, int ve
                                 /* (one read and one write of the populations per two */
                                 /* steps, the same values bit for bit) wherever a     */
                                 /* kernel exists: D3Q19 and D3Q27; x a multiple of 32; */
-                                /* one population array below 4 GiB; 0 one step per  */
+                                /* each population's array below 4 GiB; 0 one step per */
                                 /* launch; -1 (default) automatic: on for             */
                                 /* 4- and 8-byte storage on lattices of at least 2^18 */
                                 /* cells (2^19 for D3Q27).  The staged calls pair     */
