@@ -109,8 +109,11 @@ __device__ inline void gather_mask(typename Prec<V>::S (&fv)[Q], const typename 
 
 // Step t + 2 of a cell into its own slots (zeros for a cell that is not fluid).
 // Each population's base pointer is a scalar and the cell's byte offset is
-// shared by all of them, so a store takes no address arithmetic of its own
-// (one 32-bit offset per population would be a vector add per store).
+// shared by all of them.  (The compiler turns these into one 64-bit address
+// add per store, where per-population 32-bit offsets from the array base
+// kept about 30 loop-carried offsets in vector registers; forcing the base
+// into a scalar operand, as the cell kernel does, compiled this kernel to
+// more instructions.)
 template <int Q, int V, bool NT>
 __device__ inline void store_own(const typename Prec<V>::T (&f)[Q], typename Prec<V>::S* __restrict__ B, uint32_t c,
                                  uint32_t ps, bool zero) {
