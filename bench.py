@@ -236,8 +236,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU; every GPU box here has one, and the "
-              "multi-GPU slab decomposition is a design only (no N>1 figure).", file=sys.stderr)
+        print("bench.py: measures one GPU only; the multi-GPU slab decomposition is a design "
+              "(no figure for N > 1).", file=sys.stderr)
         sys.exit(2)
 
     import plbpd_amd as P
