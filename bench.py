@@ -190,7 +190,7 @@ def pattern_ceiling(capi, variant, solids, achieved):
 def cpu_baseline(sample_steps=150, single_core_steps=8):
     """About 10 s of work on all host cores and 7 s on one (at about 250 and
     20 MLUPS): a bounded sample of the same lattice, reported, not targeted."""
-    from oracle.pyoracle import CpuBaseline  # the checker side: restated reference CPU path
+    from oracle.pyoracle import CpuBaseline  # the oracle's restatement of the reference CPU path
     cb = CpuBaseline()
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     nu = (TAU - 0.5) / 3.0
@@ -352,7 +352,7 @@ def main():
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         try:
             line["cpu_baseline"] = cpu_baseline()
-        except Exception as e:  # the checker library may be missing on a stripped tree
+        except Exception as e:  # the oracle library may be missing on a stripped tree
             line["cpu_baseline"] = {"value": None, "unit": "MLUPS", "error": str(e)}
     print(json.dumps(line))
 

@@ -2,7 +2,7 @@
 //
 // A CPU restatement of the reference lattice-Boltzmann BGK step of
 // pchakraborty/plbpd, used by tests/ (and by bench.py's cpu_baseline leg) as
-// the checker for the HIP product.  Nothing in plbpd_amd/ links, loads or
+// the reference the HIP product is compared with.  Nothing in plbpd_amd/ links, loads or
 // calls this file.  Built with -ffp-contract=off so that every expression is
 // evaluated exactly in the order written below.
 //

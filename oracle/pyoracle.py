@@ -1,6 +1,6 @@
 """ctypes wrapper of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
-Loaded by tests/, by __graft_entry__.smoke() (as the checker) and by
+Loaded by tests/, by __graft_entry__.smoke() (to compare with) and by
 bench.py's cpu_baseline leg.  The product (plbpd_amd/) never imports it.
 
 Arrays follow the reference's AoS order: literal mode uses the halo'd field
